@@ -1,0 +1,140 @@
+/*
+ * sep.h -- C ABI of the MI355X (gfx950) eigensolver library libsep_amd.so.
+ *
+ * Drop-in for the hot path of SymmetricEigenProblem.jl: the QR algorithm for real
+ * symmetric tridiagonal matrices with Wilkinson shifts, and the bulk application of
+ * Givens rotations in wave order.  Every entry point takes plain pointers and 64-bit
+ * sizes (Julia's Int), matrices are column-major with an explicit leading dimension,
+ * and everything is mutated in place, as in the reference.
+ *
+ * Return codes: SEP_OK (0); SEP_NOT_CONVERGED (1) where the reference returns
+ * converged = false; negative values for invalid arguments (the reference's
+ * @assert) and HIP errors.  sep_last_error() describes the last failure of the
+ * calling thread.
+ *
+ * Pointer conventions: functions without a suffix take HOST pointers and copy
+ * through the handle's device buffers; *_dev functions take DEVICE pointers and run
+ * on the handle's stream (they return once the work is queued unless stated).
+ */
+#ifndef SEP_H
+#define SEP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SEP_OK 0
+#define SEP_NOT_CONVERGED 1
+#define SEP_ERR_ARG -1        /* invalid size / pointer / leading dimension */
+#define SEP_ERR_ODD_N -2      /* odd order: the reference's @assert iseven(n) */
+#define SEP_ERR_BAD_LAYERS -3 /* max_layers < 1 (or beyond the library limit) */
+#define SEP_ERR_HIP -4        /* a HIP runtime call failed */
+#define SEP_ERR_NOMEM -5      /* device allocation failed */
+#define SEP_ERR_UNSUPPORTED -6
+
+typedef struct sep_handle sep_handle;
+
+/* Library handle: owns the device buffers, two HIP streams (chase / apply),
+ * the autotune table and the captured batched graphs.  One handle per host thread. */
+int sep_create(sep_handle **h, int device);
+int sep_destroy(sep_handle *h);
+/* Message of the last failing call on this thread ("" if none). */
+const char *sep_last_error(void);
+/* Library / kernel identification string, e.g. "libsep_amd gfx950 ...". */
+const char *sep_version(void);
+/* Run subsequent *_dev work on `stream` (a hipStream_t; NULL = the handle's own). */
+int sep_set_stream(sep_handle *h, void *stream);
+int sep_synchronize(sep_handle *h);
+
+/* ---- Seam 2: bulk Givens application in wave order --------------------------
+ * Replaces bulk_wave_order_2x2_rmul!(A, givens) and its _parallel! variant
+ * (src/wave.jl:62 and :36; called from src/bulge_chasing.jl:282 and
+ * benchmark/benchmark.jl:13,27).
+ * A is m x n column-major (leading dimension lda >= m).  cs is the (n-1) x k grid of
+ * (c,s) pairs, column-major and interleaved, i.e. the memory of a Julia
+ * Matrix{Tuple{T,T}}: c of rotation (i, L) at cs[2*(i + (n-1)*L)], s right after
+ * (0-based i, L).  The result is A * prod_{L} prod_{i} G(i,L)^T in layer-major order,
+ * where G(i,L)^T maps columns (i, i+1): a_i' = c a_i + s a_{i+1},
+ * a_{i+1}' = -s a_i + c a_{i+1} (Rotation2 rmul!, src/simple_givens.jl:53-64).
+ * Unlike the reference there is no parity precondition on n or k. */
+int sep_bulk_rmul_f64(sep_handle *h, int64_t m, int64_t n, double *A, int64_t lda,
+                      const double *cs, int64_t k);
+int sep_bulk_rmul_f32(sep_handle *h, int64_t m, int64_t n, float *A, int64_t lda,
+                      const float *cs, int64_t k);
+int sep_bulk_rmul_f64_dev(sep_handle *h, int64_t m, int64_t n, double *A, int64_t lda,
+                          const double *cs, int64_t k);
+int sep_bulk_rmul_f32_dev(sep_handle *h, int64_t m, int64_t n, float *A, int64_t lda,
+                          const float *cs, int64_t k);
+
+/* ---- Seam 1: the whole solve ------------------------------------------------
+ * Replaces qr_algorithm!(H::SymTridiagonal{T}, Q, max_layers=128, tol=eps(T),
+ * maxiter=100n) -> (H, converged, iter)  (src/bulge_chasing.jl:146,181,287).
+ * dv (n) and ev (n-1) are H.dv / H.ev; on return dv holds the eigenvalues in the
+ * reference's deflation order and ev the (negligible) off-diagonal.  Q is n x n
+ * column-major (ldq >= n) and is multiplied from the right by all rotations, so
+ * A Q0 = Q Lambda when Q0 = I.  Pass Q = NULL for eigenvalues only.  n must be even
+ * (the reference's @assert).  *converged, *iter as the reference's tuple;
+ * *n_rotations (may be NULL) counts the rotations the chase created, which gives the
+ * algorithmic flop count 6 n R of the eigenvector update.
+ * Returns SEP_NOT_CONVERGED if iter exceeded maxiter (then, as in the reference, the
+ * rotations of the unfinished block are not applied to Q). */
+int sep_qr_algorithm_f64(sep_handle *h, int64_t n, double *dv, double *ev, double *Q,
+                         int64_t ldq, int64_t max_layers, double tol, int64_t maxiter,
+                         int32_t *converged, int64_t *iter, int64_t *n_rotations);
+int sep_qr_algorithm_f32(sep_handle *h, int64_t n, float *dv, float *ev, float *Q, int64_t ldq,
+                         int64_t max_layers, float tol, int64_t maxiter, int32_t *converged,
+                         int64_t *iter, int64_t *n_rotations);
+/* Same with device pointers; synchronises the handle before returning (the status
+ * words come back to the host). */
+int sep_qr_algorithm_f64_dev(sep_handle *h, int64_t n, double *dv, double *ev, double *Q,
+                             int64_t ldq, int64_t max_layers, double tol, int64_t maxiter,
+                             int32_t *converged, int64_t *iter, int64_t *n_rotations);
+int sep_qr_algorithm_f32_dev(sep_handle *h, int64_t n, float *dv, float *ev, float *Q,
+                             int64_t ldq, int64_t max_layers, float tol, int64_t maxiter,
+                             int32_t *converged, int64_t *iter, int64_t *n_rotations);
+
+/* ---- Batched solve (no counterpart in the reference) --------------------------
+ * B independent problems, problem p of even order n[p] stored at dv + off_d[p],
+ * ev + off_d[p] (n[p]-1 used) and Q + off_q[p] (n[p] x n[p], column-major, ld n[p]).
+ * Device pointers for dv/ev/Q/converged/iters; n/off_d/off_q are HOST arrays.
+ * Every problem runs qr_algorithm! semantics (same defaults: tol = eps,
+ * maxiter = 100 n[p]) inside one persistent launch, captured in a HIP graph that is
+ * replayed for an identical batch layout.  max_layers <= 0 lets the autotuner pick
+ * the layer depth.  converged[p] (int32) and iters[p] (int64) are written per
+ * problem.  Returns SEP_NOT_CONVERGED if any problem did not converge. */
+int sep_qr_batched_f64_dev(sep_handle *h, int64_t B, const int64_t *n, const int64_t *off_d,
+                           const int64_t *off_q, double *dv, double *ev, double *Q,
+                           int64_t max_layers, int32_t *converged, int64_t *iters);
+/* Autotune the batched kernel's launch parameters for this batch layout (timed on
+ * scratch copies; the inputs are not modified).  Writes the chosen configuration
+ * index to *choice (may be NULL).  The table lives in the handle. */
+int sep_tune_batched_f64(sep_handle *h, int64_t B, const int64_t *n, const int64_t *off_d,
+                         const int64_t *off_q, const double *dv, const double *ev,
+                         int32_t *choice);
+
+/* ---- Backward of the eigendecomposition (no counterpart in the reference) -----
+ * Given eigenvalues lam (n), eigenvectors Q (n x n, column-major, ldq) of the
+ * tridiagonal T(d, e) and the cotangents lam_bar (n) and Q_bar (n x n, ld ldq),
+ * computes d_bar (n) and e_bar (n-1):
+ *   M = diag(lam_bar) + F o (Q^T Q_bar),  F_ij = 1/(lam_j - lam_i) (0 on the diagonal)
+ *   d_bar_k = (Q M Q^T)_kk,  e_bar_k = (Q M Q^T)_{k,k+1} + (Q M Q^T)_{k+1,k}.
+ * Q^T Q_bar and Q M run on FP64 MFMA; the row projection is fused into the second
+ * product.  All pointers are device pointers. */
+int sep_eig_backward_f64_dev(sep_handle *h, int64_t n, const double *lam, const double *Q,
+                             int64_t ldq, const double *lam_bar, const double *Q_bar,
+                             double *d_bar, double *e_bar);
+/* Batched backward: problem p has order n[p], lam/lam_bar/d_bar at off_d[p], e_bar at
+ * off_d[p] (n[p]-1 used), Q/Q_bar at off_q[p] with ld n[p].  n/off_d/off_q are HOST
+ * arrays; the rest are device pointers. */
+int sep_eig_backward_batched_f64_dev(sep_handle *h, int64_t B, const int64_t *n,
+                                     const int64_t *off_d, const int64_t *off_q,
+                                     const double *lam, const double *Q, const double *lam_bar,
+                                     const double *Q_bar, double *d_bar, double *e_bar);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SEP_H */

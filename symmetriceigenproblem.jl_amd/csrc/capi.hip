@@ -1,0 +1,397 @@
+// capi.hip -- the extern "C" surface of libsep_amd (declared in include/sep.h):
+// handle lifetime, error reporting, argument validation and host<->device staging.
+#include <cstdarg>
+#include <cstring>
+
+#include "sep_internal.hpp"
+
+namespace sep {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int DevBuf::reserve(size_t want)
+{
+    if (want <= bytes && p) return SEP_OK;
+    if (p) {
+        (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    if (want == 0) return SEP_OK;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        p = nullptr;
+        set_error("hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+        return SEP_ERR_NOMEM;
+    }
+    bytes = want;
+    return SEP_OK;
+}
+
+void DevBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+}  // namespace sep
+
+using namespace sep;
+
+namespace {
+
+template <typename T>
+int bulk_rmul_dev_checked(sep_handle* h, int64_t m, int64_t n, T* A, int64_t lda, const T* cs,
+                          int64_t k)
+{
+    if (!h) {
+        set_error("sep_bulk_rmul: null handle");
+        return SEP_ERR_ARG;
+    }
+    if (m < 0 || n < 0 || k < 0 || lda < (m > 1 ? m : 1)) {
+        set_error("sep_bulk_rmul: invalid sizes m=%lld n=%lld k=%lld lda=%lld", (long long)m,
+                  (long long)n, (long long)k, (long long)lda);
+        return SEP_ERR_ARG;
+    }
+    if (m == 0 || n <= 1 || k == 0) return SEP_OK;
+    if (!A || !cs) {
+        set_error("sep_bulk_rmul: null pointer");
+        return SEP_ERR_ARG;
+    }
+    SEP_HIP(hipSetDevice(h->device));
+    return bulk_apply_dev<T>(h, h->stream, h->apply_work[0], m, n, A, lda, cs, n - 1, k);
+}
+
+template <typename T>
+int bulk_rmul_host(sep_handle* h, int64_t m, int64_t n, T* A, int64_t lda, const T* cs, int64_t k)
+{
+    if (!h) {
+        set_error("sep_bulk_rmul: null handle");
+        return SEP_ERR_ARG;
+    }
+    if (m < 0 || n < 0 || k < 0 || lda < (m > 1 ? m : 1)) {
+        set_error("sep_bulk_rmul: invalid sizes");
+        return SEP_ERR_ARG;
+    }
+    if (m == 0 || n <= 1 || k == 0) return SEP_OK;
+    if (!A || !cs) {
+        set_error("sep_bulk_rmul: null pointer");
+        return SEP_ERR_ARG;
+    }
+    SEP_HIP(hipSetDevice(h->device));
+    const size_t abytes = sizeof(T) * (size_t)m * (size_t)n;
+    const size_t gbytes = sizeof(T) * 2 * (size_t)(n - 1) * (size_t)k;
+    if (int rc = h->host_io_A.reserve(abytes)) return rc;
+    if (int rc = h->host_io_cs.reserve(gbytes)) return rc;
+    T* dA = h->host_io_A.as<T>();
+    SEP_HIP(hipMemcpy2DAsync(dA, sizeof(T) * m, A, sizeof(T) * lda, sizeof(T) * m, n,
+                             hipMemcpyHostToDevice, h->stream));
+    SEP_HIP(hipMemcpyAsync(h->host_io_cs.p, cs, gbytes, hipMemcpyHostToDevice, h->stream));
+    if (int rc = bulk_apply_dev<T>(h, h->stream, h->apply_work[0], m, n, dA, m,
+                                   h->host_io_cs.as<T>(), n - 1, k))
+        return rc;
+    SEP_HIP(hipMemcpy2DAsync(A, sizeof(T) * lda, dA, sizeof(T) * m, sizeof(T) * m, n,
+                             hipMemcpyDeviceToHost, h->stream));
+    SEP_HIP(hipStreamSynchronize(h->stream));
+    return SEP_OK;
+}
+
+template <typename T>
+int qr_checked(sep_handle* h, int64_t n, T* dv, T* ev, T* Q, int64_t ldq, int64_t max_layers,
+               T tol, int64_t maxiter, int32_t* converged, int64_t* iter, int64_t* n_rot)
+{
+    if (!h) {
+        set_error("sep_qr_algorithm: null handle");
+        return SEP_ERR_ARG;
+    }
+    if (n < 0 || (n > 0 && (!dv || (n > 1 && !ev)))) {
+        set_error("sep_qr_algorithm: invalid n or null dv/ev");
+        return SEP_ERR_ARG;
+    }
+    if (n % 2 != 0) {
+        set_error("sep_qr_algorithm: n=%lld must be even (the reference asserts iseven(n))",
+                  (long long)n);
+        return SEP_ERR_ODD_N;
+    }
+    if (max_layers < 1 || max_layers > 4096) {
+        set_error("sep_qr_algorithm: max_layers=%lld outside [1, 4096]", (long long)max_layers);
+        return SEP_ERR_BAD_LAYERS;
+    }
+    if (Q && ldq < (n > 1 ? n : 1)) {
+        set_error("sep_qr_algorithm: ldq=%lld < n=%lld", (long long)ldq, (long long)n);
+        return SEP_ERR_ARG;
+    }
+    SEP_HIP(hipSetDevice(h->device));
+    return qr_algorithm_dev<T>(h, n, dv, ev, Q, ldq, max_layers, tol, maxiter, converged, iter,
+                               n_rot);
+}
+
+template <typename T>
+int qr_host(sep_handle* h, int64_t n, T* dv, T* ev, T* Q, int64_t ldq, int64_t max_layers, T tol,
+            int64_t maxiter, int32_t* converged, int64_t* iter, int64_t* n_rot)
+{
+    if (!h) {
+        set_error("sep_qr_algorithm: null handle");
+        return SEP_ERR_ARG;
+    }
+    if (n < 0 || (n > 0 && (!dv || (n > 1 && !ev)))) {
+        set_error("sep_qr_algorithm: invalid n or null dv/ev");
+        return SEP_ERR_ARG;
+    }
+    if (n % 2 != 0) {
+        set_error("sep_qr_algorithm: n=%lld must be even (the reference asserts iseven(n))",
+                  (long long)n);
+        return SEP_ERR_ODD_N;
+    }
+    if (Q && ldq < (n > 1 ? n : 1)) {
+        set_error("sep_qr_algorithm: ldq=%lld < n=%lld", (long long)ldq, (long long)n);
+        return SEP_ERR_ARG;
+    }
+    if (n == 0) {
+        if (converged) *converged = 1;
+        if (iter) *iter = 0;
+        if (n_rot) *n_rot = 0;
+        return SEP_OK;
+    }
+    SEP_HIP(hipSetDevice(h->device));
+    if (int rc = h->host_io_d.reserve(sizeof(T) * (size_t)n)) return rc;
+    if (int rc = h->host_io_e.reserve(sizeof(T) * (size_t)(n > 1 ? n - 1 : 1))) return rc;
+    T* dd = h->host_io_d.as<T>();
+    T* de = h->host_io_e.as<T>();
+    T* dQ = nullptr;
+    SEP_HIP(hipMemcpyAsync(dd, dv, sizeof(T) * n, hipMemcpyHostToDevice, h->stream));
+    if (n > 1)
+        SEP_HIP(hipMemcpyAsync(de, ev, sizeof(T) * (n - 1), hipMemcpyHostToDevice, h->stream));
+    if (Q) {
+        if (int rc = h->host_io_A.reserve(sizeof(T) * (size_t)n * (size_t)n)) return rc;
+        dQ = h->host_io_A.as<T>();
+        SEP_HIP(hipMemcpy2DAsync(dQ, sizeof(T) * n, Q, sizeof(T) * ldq, sizeof(T) * n, n,
+                                 hipMemcpyHostToDevice, h->stream));
+    }
+    int rc = qr_checked<T>(h, n, dd, de, dQ, n, max_layers, tol, maxiter, converged, iter, n_rot);
+    if (rc < 0) return rc;
+    SEP_HIP(hipMemcpyAsync(dv, dd, sizeof(T) * n, hipMemcpyDeviceToHost, h->stream));
+    if (n > 1)
+        SEP_HIP(hipMemcpyAsync(ev, de, sizeof(T) * (n - 1), hipMemcpyDeviceToHost, h->stream));
+    if (Q)
+        SEP_HIP(hipMemcpy2DAsync(Q, sizeof(T) * ldq, dQ, sizeof(T) * n, sizeof(T) * n, n,
+                                 hipMemcpyDeviceToHost, h->stream));
+    SEP_HIP(hipStreamSynchronize(h->stream));
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sep_last_error(void) { return sep::g_err; }
+
+const char* sep_version(void)
+{
+    return "libsep_amd 0.1 gfx950 (systolic wave-order Givens apply, device Wilkinson chase, "
+           "batched persistent solver, FP64-MFMA backward)";
+}
+
+int sep_create(sep_handle** out, int device)
+{
+    if (!out) {
+        set_error("sep_create: null output pointer");
+        return SEP_ERR_ARG;
+    }
+    *out = nullptr;
+    int count = 0;
+    SEP_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) {
+        set_error("sep_create: device %d not present (%d devices)", device, count);
+        return SEP_ERR_ARG;
+    }
+    SEP_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    SEP_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        set_error("sep_create: device %d is %s; this library is built for gfx950 only", device,
+                  prop.gcnArchName);
+        return SEP_ERR_UNSUPPORTED;
+    }
+    sep_handle* h = new sep_handle();
+    h->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->chase_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_a, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_b, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        set_error("sep_create: %s", hipGetErrorString(e));
+        sep_destroy(h);
+        return SEP_ERR_HIP;
+    }
+    *out = h;
+    return SEP_OK;
+}
+
+int sep_destroy(sep_handle* h)
+{
+    if (!h) return SEP_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->chase_stream) (void)hipStreamSynchronize(h->chase_stream);
+    if (h->batched_exec) (void)hipGraphExecDestroy(h->batched_exec);
+    h->host_io_A.release();
+    h->host_io_cs.release();
+    h->host_io_d.release();
+    h->host_io_e.release();
+    for (auto& w : h->apply_work) w.table.release();
+    for (auto& g : h->grid) g.release();
+    h->state.release();
+    h->counters.release();
+    h->lt.release();
+    h->batch_meta.release();
+    h->batch_scratch.release();
+    h->bwd_W.release();
+    if (h->ev_a) (void)hipEventDestroy(h->ev_a);
+    if (h->ev_b) (void)hipEventDestroy(h->ev_b);
+    if (h->chase_stream) (void)hipStreamDestroy(h->chase_stream);
+    if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return SEP_OK;
+}
+
+int sep_set_stream(sep_handle* h, void* stream)
+{
+    if (!h) {
+        set_error("sep_set_stream: null handle");
+        return SEP_ERR_ARG;
+    }
+    SEP_HIP(hipSetDevice(h->device));
+    if (h->own_stream && h->stream) SEP_HIP(hipStreamSynchronize(h->stream));
+    if (stream) {
+        if (h->own_stream && h->stream) SEP_HIP(hipStreamDestroy(h->stream));
+        h->stream = static_cast<hipStream_t>(stream);
+        h->own_stream = false;
+    } else if (!h->own_stream) {
+        SEP_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        h->own_stream = true;
+    }
+    return SEP_OK;
+}
+
+int sep_synchronize(sep_handle* h)
+{
+    if (!h) {
+        set_error("sep_synchronize: null handle");
+        return SEP_ERR_ARG;
+    }
+    SEP_HIP(hipSetDevice(h->device));
+    SEP_HIP(hipStreamSynchronize(h->stream));
+    SEP_HIP(hipStreamSynchronize(h->chase_stream));
+    return SEP_OK;
+}
+
+int sep_bulk_rmul_f64(sep_handle* h, int64_t m, int64_t n, double* A, int64_t lda,
+                      const double* cs, int64_t k)
+{
+    return bulk_rmul_host<double>(h, m, n, A, lda, cs, k);
+}
+int sep_bulk_rmul_f32(sep_handle* h, int64_t m, int64_t n, float* A, int64_t lda, const float* cs,
+                      int64_t k)
+{
+    return bulk_rmul_host<float>(h, m, n, A, lda, cs, k);
+}
+int sep_bulk_rmul_f64_dev(sep_handle* h, int64_t m, int64_t n, double* A, int64_t lda,
+                          const double* cs, int64_t k)
+{
+    return bulk_rmul_dev_checked<double>(h, m, n, A, lda, cs, k);
+}
+int sep_bulk_rmul_f32_dev(sep_handle* h, int64_t m, int64_t n, float* A, int64_t lda,
+                          const float* cs, int64_t k)
+{
+    return bulk_rmul_dev_checked<float>(h, m, n, A, lda, cs, k);
+}
+
+int sep_qr_algorithm_f64(sep_handle* h, int64_t n, double* dv, double* ev, double* Q,
+                         int64_t ldq, int64_t max_layers, double tol, int64_t maxiter,
+                         int32_t* converged, int64_t* iter, int64_t* n_rotations)
+{
+    return qr_host<double>(h, n, dv, ev, Q, ldq, max_layers, tol, maxiter, converged, iter,
+                           n_rotations);
+}
+int sep_qr_algorithm_f32(sep_handle* h, int64_t n, float* dv, float* ev, float* Q, int64_t ldq,
+                         int64_t max_layers, float tol, int64_t maxiter, int32_t* converged,
+                         int64_t* iter, int64_t* n_rotations)
+{
+    return qr_host<float>(h, n, dv, ev, Q, ldq, max_layers, tol, maxiter, converged, iter,
+                          n_rotations);
+}
+int sep_qr_algorithm_f64_dev(sep_handle* h, int64_t n, double* dv, double* ev, double* Q,
+                             int64_t ldq, int64_t max_layers, double tol, int64_t maxiter,
+                             int32_t* converged, int64_t* iter, int64_t* n_rotations)
+{
+    return qr_checked<double>(h, n, dv, ev, Q, ldq, max_layers, tol, maxiter, converged, iter,
+                              n_rotations);
+}
+int sep_qr_algorithm_f32_dev(sep_handle* h, int64_t n, float* dv, float* ev, float* Q,
+                             int64_t ldq, int64_t max_layers, float tol, int64_t maxiter,
+                             int32_t* converged, int64_t* iter, int64_t* n_rotations)
+{
+    return qr_checked<float>(h, n, dv, ev, Q, ldq, max_layers, tol, maxiter, converged, iter,
+                             n_rotations);
+}
+
+int sep_qr_batched_f64_dev(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d,
+                           const int64_t* off_q, double* dv, double* ev, double* Q,
+                           int64_t max_layers, int32_t* converged, int64_t* iters)
+{
+    if (!h) {
+        set_error("sep_qr_batched: null handle");
+        return SEP_ERR_ARG;
+    }
+    SEP_HIP(hipSetDevice(h->device));
+    return qr_batched_f64(h, B, n, off_d, off_q, dv, ev, Q, max_layers, converged, iters);
+}
+
+int sep_tune_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d,
+                         const int64_t* off_q, const double* dv, const double* ev,
+                         int32_t* choice)
+{
+    if (!h) {
+        set_error("sep_tune_batched: null handle");
+        return SEP_ERR_ARG;
+    }
+    SEP_HIP(hipSetDevice(h->device));
+    return tune_batched_f64(h, B, n, off_d, off_q, dv, ev, choice);
+}
+
+int sep_eig_backward_f64_dev(sep_handle* h, int64_t n, const double* lam, const double* Q,
+                             int64_t ldq, const double* lam_bar, const double* Q_bar,
+                             double* d_bar, double* e_bar)
+{
+    if (!h) {
+        set_error("sep_eig_backward: null handle");
+        return SEP_ERR_ARG;
+    }
+    SEP_HIP(hipSetDevice(h->device));
+    return eig_backward_f64(h, n, lam, Q, ldq, lam_bar, Q_bar, d_bar, e_bar);
+}
+
+int sep_eig_backward_batched_f64_dev(sep_handle* h, int64_t B, const int64_t* n,
+                                     const int64_t* off_d, const int64_t* off_q,
+                                     const double* lam, const double* Q, const double* lam_bar,
+                                     const double* Q_bar, double* d_bar, double* e_bar)
+{
+    if (!h) {
+        set_error("sep_eig_backward_batched: null handle");
+        return SEP_ERR_ARG;
+    }
+    SEP_HIP(hipSetDevice(h->device));
+    return eig_backward_batched_f64(h, B, n, off_d, off_q, lam, Q, lam_bar, Q_bar, d_bar, e_bar);
+}
+
+}  // extern "C"

@@ -1,0 +1,167 @@
+// chase.hip -- qr_algorithm! (src/bulge_chasing.jl:146-288) on one GPU.
+//
+// Host loop over blocks of at most max_layers sweeps.  Block b is chased by a
+// one-wavefront kernel on the handle's chase stream into grid buffer b % 2, then
+// applied to Q by the systolic apply on the apply stream, while block b+1 is already
+// being chased into the other grid buffer.  The host reads the 16-word block state
+// (pinned) after each chase to decide whether the solve continues, exactly where the
+// reference's outer loop tests did_a_thing / iter > maxiter.
+#include "chase.hpp"
+#include "sep_internal.hpp"
+
+namespace sep {
+namespace {
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill_identity(T* __restrict__ grid, int64_t pairs)
+{
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < pairs;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        grid[2 * q] = T(1);
+        grid[2 * q + 1] = T(0);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_chase_block(T* dv, T* ev, int* lt, T* grid, int n,
+                                                    int max_layers, T tol, int64_t maxiter,
+                                                    int64_t* st)
+{
+    chase::Problem<T> P{dv, ev, lt, grid, n, max_layers, (int64_t)(n - 1)};
+    chase::chase_block(P, tol, maxiter, st, (int)threadIdx.x);
+}
+
+template <typename T>
+int fill_identity(hipStream_t s, T* grid, int64_t pairs)
+{
+    const int blocks = (int)std::min<int64_t>((pairs + 255) / 256, 4096);
+    hipLaunchKernelGGL((k_fill_identity<T>), dim3(blocks), dim3(256), 0, s, grid, pairs);
+    SEP_CHECK_LAUNCH();
+    return SEP_OK;
+}
+
+struct PinnedState {
+    int64_t* p = nullptr;
+    ~PinnedState()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+}  // namespace
+
+template <typename T>
+int qr_algorithm_dev(sep_handle* h, int64_t n64, T* dv, T* ev, T* Q, int64_t ldq,
+                     int64_t max_layers, T tol, int64_t maxiter, int32_t* converged,
+                     int64_t* iter_out, int64_t* nrot_out)
+{
+    if (n64 > INT32_MAX / 2) {
+        set_error("qr_algorithm: n=%lld too large", (long long)n64);
+        return SEP_ERR_ARG;
+    }
+    const int n = (int)n64;
+    if (n <= 1) {  // nothing to chase (the reference's loop breaks at once)
+        if (converged) *converged = 1;
+        if (iter_out) *iter_out = 0;
+        if (nrot_out) *nrot_out = 0;
+        return SEP_OK;
+    }
+    hipStream_t cs = h->chase_stream, as = h->stream;
+    const int64_t pairs = (int64_t)(n - 1) * max_layers;
+    for (int b = 0; b < 2; ++b)
+        if (int rc = h->grid[b].reserve(sizeof(T) * 2 * (size_t)pairs)) return rc;
+    if (int rc = h->lt.reserve(sizeof(int) * (size_t)n)) return rc;
+    if (int rc = h->state.reserve(sizeof(int64_t) * chase::ST_WORDS)) return rc;
+    PinnedState hs;
+    SEP_HIP(hipHostMalloc((void**)&hs.p, sizeof(int64_t) * chase::ST_WORDS, hipHostMallocDefault));
+    int64_t* dst = h->state.as<int64_t>();
+
+    // the chase must see dv/ev as the caller's stream left them
+    SEP_HIP(hipEventRecord(h->ev_b, as));
+    SEP_HIP(hipStreamWaitEvent(cs, h->ev_b, 0));
+
+    int64_t init[chase::ST_WORDS] = {0};
+    init[chase::ST_TO] = n;
+    init[chase::ST_LBEA] = n;
+    init[chase::ST_ITER] = 0;
+    std::memcpy(hs.p, init, sizeof(init));
+    SEP_HIP(hipMemcpyAsync(dst, hs.p, sizeof(init), hipMemcpyHostToDevice, cs));
+
+    hipEvent_t applied[2] = {nullptr, nullptr};
+    hipEvent_t chased = nullptr;
+    SEP_HIP(hipEventCreateWithFlags(&applied[0], hipEventDisableTiming));
+    SEP_HIP(hipEventCreateWithFlags(&applied[1], hipEventDisableTiming));
+    SEP_HIP(hipEventCreateWithFlags(&chased, hipEventDisableTiming));
+    bool applied_pending[2] = {false, false};
+
+    int rc = SEP_OK;
+    int64_t total_rot = 0, iter = 0;
+    int conv = 1;
+    int cur = 0;
+    for (;;) {
+        T* grid = h->grid[cur].as<T>();
+        if (applied_pending[cur]) {
+            // the apply of two blocks ago still reads this grid buffer
+            if (hipStreamWaitEvent(cs, applied[cur], 0) != hipSuccess) { rc = SEP_ERR_HIP; break; }
+        }
+        if ((rc = fill_identity<T>(cs, grid, pairs))) break;
+        hipLaunchKernelGGL((k_chase_block<T>), dim3(1), dim3(64), 0, cs, dv, ev,
+                           h->lt.as<int>(), grid, n, (int)max_layers, tol, maxiter, dst);
+        if (hipGetLastError() != hipSuccess) {
+            set_error("chase launch failed");
+            rc = SEP_ERR_HIP;
+            break;
+        }
+        if (hipMemcpyAsync(hs.p, dst, sizeof(int64_t) * chase::ST_WORDS, hipMemcpyDeviceToHost,
+                           cs) != hipSuccess ||
+            hipEventRecord(chased, cs) != hipSuccess ||
+            hipEventSynchronize(chased) != hipSuccess) {
+            set_error("chase: state read-back failed: %s", hipGetErrorString(hipGetLastError()));
+            rc = SEP_ERR_HIP;
+            break;
+        }
+        const int64_t* st = hs.p;
+        iter = st[chase::ST_ITER];
+        if (st[chase::ST_EARLY]) {  // :180-182 -- return without applying this block
+            conv = 0;
+            break;
+        }
+        if (!st[chase::ST_DID]) break;  // :279
+        total_rot += st[chase::ST_NROT];
+        if (Q) {
+            const int64_t ilo = st[chase::ST_ILO], ihi = st[chase::ST_IHI];
+            const int64_t depth = st[chase::ST_MAXLAYER];
+            if (hipStreamWaitEvent(as, chased, 0) != hipSuccess) { rc = SEP_ERR_HIP; break; }
+            // rotations ilo..ihi (1-based) touch columns ilo-1 .. ihi (0-based)
+            rc = bulk_apply_dev<T>(h, as, h->apply_work[cur], n, ihi - ilo + 2,
+                                   Q + (ilo - 1) * ldq, ldq, grid + 2 * (ilo - 1), n - 1, depth);
+            if (rc) break;
+            if (hipEventRecord(applied[cur], as) != hipSuccess) { rc = SEP_ERR_HIP; break; }
+            applied_pending[cur] = true;
+        }
+        cur ^= 1;
+    }
+    // the caller's stream (apply stream) must also see the final dv/ev
+    (void)hipEventRecord(chased, cs);
+    (void)hipStreamWaitEvent(as, chased, 0);
+    hipError_t se = hipStreamSynchronize(as);
+    if (se != hipSuccess && rc == SEP_OK) {
+        set_error("qr_algorithm: %s", hipGetErrorString(se));
+        rc = SEP_ERR_HIP;
+    }
+    (void)hipEventDestroy(applied[0]);
+    (void)hipEventDestroy(applied[1]);
+    (void)hipEventDestroy(chased);
+    if (rc) return rc;
+    if (converged) *converged = conv;
+    if (iter_out) *iter_out = iter;
+    if (nrot_out) *nrot_out = total_rot;
+    return conv ? SEP_OK : SEP_NOT_CONVERGED;
+}
+
+template int qr_algorithm_dev<double>(sep_handle*, int64_t, double*, double*, double*, int64_t,
+                                      int64_t, double, int64_t, int32_t*, int64_t*, int64_t*);
+template int qr_algorithm_dev<float>(sep_handle*, int64_t, float*, float*, float*, int64_t,
+                                     int64_t, float, int64_t, int32_t*, int64_t*, int64_t*);
+
+}  // namespace sep

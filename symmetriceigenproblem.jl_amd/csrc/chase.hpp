@@ -1,0 +1,389 @@
+// chase.hpp -- device-side Wilkinson-shift bulge chasing (one wavefront per problem).
+//
+// Restates, as wave-level device code, the inner loop of qr_algorithm!
+// (src/bulge_chasing.jl:178-276): deflation scan, block-capacity check, scaled
+// Wilkinson shift and the single_shift! sweep (src/bulge_chasing.jl:11-144) that
+// records each rotation at layer ++layer_thickness[i] of the (c,s) grid.
+//
+// The sweep is a serial recurrence; the wavefront's 64 lanes are used around it:
+//  * the deflation scan and the capacity reduction test 64 indices per step (ballot /
+//    lane max);
+//  * a sweep is processed in chunks of 64 steps: lane j preloads dv[i+1], ev[i+1] and
+//    layer_thickness[i] of step i = i0 + j (values no earlier step of the sweep
+//    changes) for the next chunk while the current one runs, the recurrence reads them
+//    with v_readlane and deposits its finished outputs (ev[i-1], dv[i], c, s) back in
+//    lane j with a lane select, and the chunk is stored with one coalesced store per
+//    array.  The recurrence itself therefore never waits on memory.
+// Arithmetic follows the reference operation by operation without contraction, and
+// uses IEEE division and square root, so the grid, dv, ev and iteration count equal
+// the C restatement's bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sep {
+namespace chase {
+
+constexpr int kWave = 64;
+
+template <typename T>
+struct Consts;
+template <>
+struct Consts<double> {
+    static constexpr double safmn2 = 0x1p-485;
+    static constexpr double safmx2 = 0x1p485;
+};
+template <>
+struct Consts<float> {
+    static constexpr float safmn2 = 0x1p-51f;
+    static constexpr float safmx2 = 0x1p51f;
+};
+
+__device__ __forceinline__ double rdlane(double v, int j)
+{
+    const uint64_t u = __double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)u, j);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), j);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ float rdlane(float v, int j)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ __forceinline__ int rdlane(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
+
+__device__ __forceinline__ double tabs(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float tabs(float x) { return __builtin_fabsf(x); }
+
+#pragma clang fp contract(off)
+
+// LinearAlgebra.givensAlgorithm (classic DLARTG), via get_rotation
+// (src/simple_givens.jl:20-23).  g == 0 -> (1,0,f); f == 0 -> (0,1,g); otherwise
+// c = f/r, s = g/r, r = sqrt(f^2+g^2), negated when |f| > |g| and c < 0, with
+// power-of-two rescaling when max(|f|,|g|) leaves [2^-485, 2^485] (Float64).
+template <typename T>
+__device__ __attribute__((noinline)) void givens_rescaled(T f, T g, T& c, T& s, T& r)
+{
+    const T safmn2 = Consts<T>::safmn2, safmx2 = Consts<T>::safmx2;
+    T f1 = f, g1 = g;
+    T scale = tabs(f1) > tabs(g1) ? tabs(f1) : tabs(g1);
+    int count = 0;
+    if (scale >= safmx2) {
+        do {
+            count++;
+            f1 *= safmn2;
+            g1 *= safmn2;
+            scale = tabs(f1) > tabs(g1) ? tabs(f1) : tabs(g1);
+        } while (scale >= safmx2 && count < 20);
+        r = sqrt(f1 * f1 + g1 * g1);
+        c = f1 / r;
+        s = g1 / r;
+        for (int i = 0; i < count; i++) r *= safmx2;
+    } else {
+        do {
+            count++;
+            f1 *= safmx2;
+            g1 *= safmx2;
+            scale = tabs(f1) > tabs(g1) ? tabs(f1) : tabs(g1);
+        } while (scale <= safmn2);
+        r = sqrt(f1 * f1 + g1 * g1);
+        c = f1 / r;
+        s = g1 / r;
+        for (int i = 0; i < count; i++) r *= safmn2;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void givens(T f, T g, T& c, T& s, T& r)
+{
+    const T af = tabs(f), ag = tabs(g);
+    const T scale = af > ag ? af : ag;
+    if (scale >= Consts<T>::safmx2 || scale <= Consts<T>::safmn2) {
+        if (g == T(0)) {
+            c = T(1); s = T(0); r = f;
+            return;
+        }
+        if (f == T(0)) {
+            c = T(0); s = T(1); r = g;
+            return;
+        }
+        givens_rescaled(f, g, c, s, r);
+    } else {
+        r = sqrt(f * f + g * g);
+        c = f / r;
+        s = g / r;
+        if (g == T(0)) {  // also covers f == g == 0 (then scale <= safmn2 anyway)
+            c = T(1); s = T(0); r = f;
+        } else if (f == T(0)) {
+            c = T(0); s = T(1); r = g;
+        }
+    }
+    if (af > ag && c < T(0)) {
+        c = -c; s = -s; r = -r;
+    }
+}
+
+// is_offdiagonal_small, src/bulge_chasing.jl:7-8 (1-based i; dv, ev 0-based arrays)
+template <typename T>
+__device__ __forceinline__ bool is_small(T e_i, T d_i, T d_ip1, T tol)
+{
+    return tabs(e_i) <= tol * (tabs(d_i) + tabs(d_ip1));
+}
+
+// Scaled Wilkinson shift of the trailing 2x2 (src/bulge_chasing.jl:239-265).
+template <typename T>
+__device__ __forceinline__ T wilkinson(T H11, T H12, T H22)
+{
+    const T scale = tabs(H11) + T(2) * tabs(H12) + tabs(H22);
+    H11 /= scale;
+    H12 /= scale;
+    H22 /= scale;
+    const T t = (H11 + H22) / T(2);
+    const T d = (H11 - t) * (H22 - t) - H12 * H12;
+    const T sq = sqrt(tabs(d));
+    const T l1 = t + sq, l2 = t - sq;
+    T lam = tabs(H22 - l1) < tabs(H22 - l2) ? l1 : l2;
+    lam *= scale;
+    return lam;
+}
+
+// Problem view.  dv (n), ev (n-1), lt (n), grid (n-1) x max_layers interleaved pairs
+// with leading dimension ldg pairs.  Indices in the member functions are 1-based.
+template <typename T>
+struct Problem {
+    T* dv;
+    T* ev;
+    int* lt;
+    T* grid;
+    int n;
+    int max_layers;
+    int64_t ldg;
+};
+
+// The highest 1-based q < to with is_small(q), scanning down to `start`, or start-1.
+template <typename T>
+__device__ int deflation_scan(const Problem<T>& P, int to, int start, T tol, int lane)
+{
+    for (int top = to - 1; top >= start; top -= kWave) {
+        const int q = top - lane;
+        bool sm = false;
+        if (q >= start) sm = is_small(P.ev[q - 1], P.dv[q - 1], P.dv[q], tol);
+        const uint64_t mask = __ballot(sm);
+        if (mask) return top - (int)__builtin_ctzll(mask);
+    }
+    return start - 1;
+}
+
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+template <typename T>
+__device__ int max_thickness(const Problem<T>& P, int from, int to, int lane)
+{
+    int v = 0;
+    for (int q = from + lane; q <= to; q += kWave) v = max(v, P.lt[q - 1]);
+    return wave_max(v);
+}
+
+// single_shift!(H, from, to, mu, givens, layer_thickness), src/bulge_chasing.jl:11-144,
+// executed by the whole wave (all lanes compute the uniform recurrence; lane j owns the
+// memory traffic of step i0 + j in each 64-step chunk).
+template <typename T>
+__device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int lane)
+{
+    T* __restrict__ dv = P.dv;
+    T* __restrict__ ev = P.ev;
+    int* __restrict__ lt = P.lt;
+    T* __restrict__ grid = P.grid;
+    const int64_t ldg = P.ldg;
+    T c, s, nrm;
+
+    // ---- first rotation, :17-63
+    const T H11 = dv[from - 1];
+    const T H21 = ev[from - 1];
+    const T H22 = dv[from];
+    givens(H11 - mu, H21, c, s, nrm);
+    const T H11p = c * H11 + s * H21;
+    const T H21p = -s * H11 + c * H21;
+    const T H12p = c * H21 + s * H22;
+    const T H22p = -s * H21 + c * H22;
+    const T H11pp = H11p * c + H12p * s;
+    const T H12pp = H11p * -s + H12p * c;
+    const T H22pp = H21p * -s + H22p * c;
+    T bulge = T(0);
+    T e_cur = T(0);  // ev[from+1] after the first step
+    if (from + 2 <= to) {
+        const T H23 = ev[from];
+        bulge = s * H23;
+        e_cur = c * H23;
+    }
+    {
+        const int layer = lt[from - 1] + 1;
+        if (lane == 0) {
+            dv[from - 1] = H11pp;
+            lt[from - 1] = layer;
+            grid[2 * ((int64_t)(from - 1) + ldg * (layer - 1))] = c;
+            grid[2 * ((int64_t)(from - 1) + ldg * (layer - 1)) + 1] = s;
+        }
+    }
+    // carried state entering step i: p1 = ev[i-1], d_cur = dv[i], e_cur = ev[i]
+    T p1 = H12pp;
+    T d_cur = H22pp;
+
+    // ---- bulge chase, :91-140, in chunks of 64 steps
+    const int i_first = from + 1, i_last = to - 1;
+    if (i_first <= i_last) {
+        // prefetch chunk 0: lane j <-> step i = i_first + j
+        T nD = T(0), nE = T(0);
+        int nL = 0;
+        {
+            const int i = i_first + lane;
+            if (i <= i_last) {
+                nD = dv[i];                      // dv[i+1] (1-based)
+                if (i + 2 <= to) nE = ev[i];     // ev[i+1] (1-based)
+                nL = lt[i - 1];
+            }
+        }
+        for (int i0 = i_first; i0 <= i_last; i0 += kWave) {
+            const T D = nD, E = nE;
+            const int Lt = nL;
+            {  // prefetch next chunk
+                const int i = i0 + kWave + lane;
+                nD = T(0);
+                nE = T(0);
+                nL = 0;
+                if (i <= i_last) {
+                    nD = dv[i];
+                    if (i + 2 <= to) nE = ev[i];
+                    nL = lt[i - 1];
+                }
+            }
+            const int cnt = min(kWave, i_last - i0 + 1);
+            T oN = T(0), oD = T(0), oC = T(0), oS = T(0);
+            for (int j = 0; j < cnt; ++j) {
+                const bool mine = lane == j;  // lane j keeps step i0 + j's outputs
+                const int i = i0 + j;
+                const T B22 = rdlane(D, j);
+                givens(p1, bulge, c, s, nrm);
+                const T B11 = d_cur, B21 = e_cur;
+                const T B11p = c * B11 + s * B21;
+                const T B21p = -s * B11 + c * B21;
+                const T B12p = c * B21 + s * B22;
+                const T B22p = -s * B21 + c * B22;
+                const T B11pp = B11p * c + B12p * s;
+                const T B12pp = B11p * -s + B12p * c;
+                const T B22pp = B21p * -s + B22p * c;
+                oN = mine ? nrm : oN;
+                oD = mine ? B11pp : oD;
+                oC = mine ? c : oC;
+                oS = mine ? s : oS;
+                p1 = B12pp;
+                d_cur = B22pp;
+                if (i + 2 <= to) {
+                    const T B23 = rdlane(E, j);
+                    bulge = s * B23;
+                    e_cur = c * B23;
+                }
+            }
+            if (lane < cnt) {
+                const int i = i0 + lane;
+                ev[i - 2] = oN;  // ev[i-1] (1-based) = nrm
+                dv[i - 1] = oD;  // dv[i]   (1-based) = B11''
+                const int layer = Lt + 1;
+                lt[i - 1] = layer;
+                grid[2 * ((int64_t)(i - 1) + ldg * (layer - 1))] = oC;
+                grid[2 * ((int64_t)(i - 1) + ldg * (layer - 1)) + 1] = oS;
+            }
+        }
+    }
+    // final carried values: dv[to] and ev[to-1] (1-based); ev[to] untouched
+    if (lane == 0) {
+        dv[to - 1] = d_cur;
+        ev[to - 2] = p1;
+    }
+}
+
+// Block state exchanged with the host (int64 words).
+enum StateWord {
+    ST_TO = 0,         // in: `to` of the next block; out: last_block_ends_at
+    ST_LBEA = 1,       // last_block_ends_at
+    ST_ITER = 2,       // iteration count (in/out)
+    ST_DID = 3,        // out: a sweep ran in this block
+    ST_EARLY = 4,      // out: iter > maxiter was hit (not converged)
+    ST_NROT = 5,       // out: rotations created in this block
+    ST_MAXLAYER = 6,   // out: largest layer index used (apply depth)
+    ST_ILO = 7,        // out: smallest rotation index used (1-based)
+    ST_IHI = 8,        // out: largest rotation index used (1-based)
+    ST_WORDS = 16
+};
+
+// One outer iteration of qr_algorithm! without the apply (src/bulge_chasing.jl:166-276):
+// layer_thickness is reset here; the grid must hold identity rotations on entry.
+template <typename T>
+__device__ void chase_block(const Problem<T>& P, T tol, int64_t maxiter, int64_t* st, int lane)
+{
+    const int start = 1;
+    int to = (int)st[ST_TO];
+    int lbea = (int)st[ST_LBEA];
+    int64_t iter = st[ST_ITER];
+    for (int q = lane; q < P.n; q += kWave) P.lt[q] = 0;
+    __threadfence_block();
+    int did = 0, early = 0;
+    int64_t nrot = 0;
+    int ilo = P.n, ihi = 0;
+    for (;;) {
+        if (iter > maxiter) {
+            early = 1;
+            break;
+        }
+        int from = to;
+        if (from > start) from = deflation_scan(P, to, start, tol, lane) + 1;
+        if (to <= start) break;
+        if (from == to) {
+            if (lane == 0) P.ev[from - 2] = T(0);
+            __threadfence_block();
+            if (to == lbea) lbea -= 1;
+            to -= 1;
+        } else {
+            const int thick = max_thickness(P, from, to, lane);
+            if (thick == P.max_layers) {
+                to = from - 1;
+            } else {
+                const T lam = wilkinson(P.dv[to - 2], P.ev[to - 2], P.dv[to - 1]);
+                did = 1;
+                single_shift(P, from, to, lam, lane);
+                // the next scan and sweep read what this sweep's lanes stored
+                __threadfence_block();
+                nrot += to - from;
+                iter += 1;
+                ilo = min(ilo, from);
+                ihi = max(ihi, to - 1);
+            }
+        }
+        if (to <= start) break;
+    }
+    int maxlayer = 0;
+    if (did) maxlayer = max_thickness(P, 1, P.n, lane);
+    if (lane == 0) {
+        st[ST_TO] = lbea;
+        st[ST_LBEA] = lbea;
+        st[ST_ITER] = iter;
+        st[ST_DID] = did;
+        st[ST_EARLY] = early;
+        st[ST_NROT] = nrot;
+        st[ST_MAXLAYER] = maxlayer;
+        st[ST_ILO] = ilo;
+        st[ST_IHI] = ihi;
+    }
+}
+
+#pragma clang fp contract(on)
+
+}  // namespace chase
+}  // namespace sep

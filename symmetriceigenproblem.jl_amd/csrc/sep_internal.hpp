@@ -1,0 +1,119 @@
+// sep_internal.hpp -- shared declarations of libsep_amd (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/sep.h"
+
+namespace sep {
+
+// Thread-local message behind sep_last_error().
+void set_error(const char* fmt, ...);
+
+#define SEP_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            ::sep::set_error("%s:%d %s -> %s", __FILE__, __LINE__, #call,               \
+                             hipGetErrorString(e_));                                    \
+            return SEP_ERR_HIP;                                                         \
+        }                                                                               \
+    } while (0)
+
+#define SEP_CHECK_LAUNCH()                                                              \
+    do {                                                                                \
+        hipError_t e_ = hipGetLastError();                                              \
+        if (e_ != hipSuccess) {                                                         \
+            ::sep::set_error("%s:%d kernel launch -> %s", __FILE__, __LINE__,           \
+                             hipGetErrorString(e_));                                    \
+            return SEP_ERR_HIP;                                                         \
+        }                                                                               \
+    } while (0)
+
+// A growable device buffer owned by the handle.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int reserve(size_t want);  // keeps contents only if no growth
+    void release();
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// Apply-kernel geometry: a pass applies KG layers in each of G waves (G*KG layers),
+// S time steps per barrier interval.
+struct ApplyGeom {
+    int KG, G, S;
+    int layers() const { return KG * G; }
+};
+
+// Per-launch workspace of the systolic apply: the relaid (c,s) sweep table.
+struct ApplyWork {
+    DevBuf table;
+};
+
+// Batched-solver tuning entry (chosen by the autotuner).
+struct BatchedTune {
+    int rows_per_lane;   // rows of Q each lane owns in the in-wave apply (1, 2, 4)
+    int fused_depth;     // rotation layers fused per pass over Q
+    int sweep_block;     // max_layers: sweeps accumulated per block
+};
+
+}  // namespace sep
+
+struct sep_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;        // apply stream (user-visible)
+    hipStream_t chase_stream = nullptr;  // chase stream (overlap with apply)
+    bool own_stream = true;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    // scratch
+    sep::DevBuf host_io_A, host_io_cs, host_io_d, host_io_e;
+    sep::ApplyWork apply_work[2];
+    sep::DevBuf grid[2], state, counters, lt;
+    sep::DevBuf batch_meta, batch_scratch;
+    sep::DevBuf bwd_W;
+    // batched graph cache
+    hipGraphExec_t batched_exec = nullptr;
+    std::vector<int64_t> batched_key;
+    int batched_choice = -1;
+    std::vector<int64_t> tune_key;
+    int tune_choice = -1;
+};
+
+namespace sep {
+
+// apply.hip
+template <typename T>
+int bulk_apply_dev(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T* A,
+                   int64_t lda, const T* cs, int64_t ld_cs, int64_t k);
+
+// chase.hip
+template <typename T>
+int qr_algorithm_dev(sep_handle* h, int64_t n, T* dv, T* ev, T* Q, int64_t ldq,
+                     int64_t max_layers, T tol, int64_t maxiter, int32_t* converged,
+                     int64_t* iter, int64_t* n_rot);
+
+// batched.hip
+int qr_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d,
+                   const int64_t* off_q, double* dv, double* ev, double* Q, int64_t max_layers,
+                   int32_t* converged, int64_t* iters);
+int tune_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d,
+                     const int64_t* off_q, const double* dv, const double* ev, int32_t* choice);
+
+// backward.hip
+int eig_backward_f64(sep_handle* h, int64_t n, const double* lam, const double* Q, int64_t ldq,
+                     const double* lam_bar, const double* Q_bar, double* d_bar, double* e_bar);
+int eig_backward_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d,
+                             const int64_t* off_q, const double* lam, const double* Q,
+                             const double* lam_bar, const double* Q_bar, double* d_bar,
+                             double* e_bar);
+
+}  // namespace sep

@@ -1,0 +1,156 @@
+"""GPU parity of qr_algorithm! (src/bulge_chasing.jl:146-288) against the oracle.
+
+The device chase follows the reference without FMA contraction and with IEEE
+division/sqrt, like the oracle, so iteration counts, deflation order and eigenvalues are
+expected to agree exactly; the tests assert the documented tolerances and report
+whether the agreement is bitwise.  Eigenvectors: Q agrees with the oracle's Q
+(the same rotations, applied with contraction) and ||AQ - Q Lam|| / ||A|| <= 1e-13,
+||Q^T Q - I|| <= 1e-13 n.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _dense(d, e):
+    return np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+
+
+def _solve_both(sep, handle, d, e, max_layers, dtype=np.float64, maxiter=None):
+    n = d.shape[0]
+    od, oe = d.astype(dtype), e.astype(dtype)
+    oQ = np.asfortranarray(np.eye(n, dtype=dtype))
+    oconv, oit, onrot, _ = oracle.qr_algorithm(od, oe, oQ, max_layers, maxiter=maxiter)
+    H = sep.SymTridiagonal(d.astype(dtype), e.astype(dtype))
+    Q = np.asfortranarray(np.eye(n, dtype=dtype))
+    _, conv, it, nrot = sep.qr_algorithm_(H, Q, max_layers, maxiter=maxiter, handle=handle,
+                                         return_rotations=True)
+    return (od, oe, oQ, oconv, oit, onrot), (H.dv, H.ev, Q, conv, it, nrot)
+
+
+@pytest.mark.parametrize("n,max_layers", [(2, 128), (4, 2), (10, 8), (64, 128), (200, 128),
+                                          (256, 64), (500, 128), (130, 1), (98, 7)])
+def test_qr_algorithm_matches_oracle(sep, handle, n, max_layers):
+    d, e = sep.inputs.tridiag_linear(n, seed=n)
+    (od, oe, oQ, oconv, oit, onrot), (gd, ge, gQ, conv, it, nrot) = _solve_both(
+        sep, handle, d, e, max_layers)
+    assert conv and oconv
+    assert it == oit and nrot == onrot
+    scale = np.abs(od).max()
+    assert np.abs(gd - od).max() <= 1e-12 * scale
+    assert np.abs(ge - oe).max() <= 1e-12 * scale
+    assert np.linalg.norm(gQ - oQ) <= 1e-13 * np.sqrt(n)
+    A = _dense(d, e)
+    assert np.linalg.norm(A @ gQ - gQ * gd) <= 1e-13 * np.linalg.norm(A)
+    assert np.linalg.norm(gQ.T @ gQ - np.eye(n)) <= 1e-13 * n
+    ref = sl.eigvalsh_tridiagonal(d, e)
+    assert np.abs(np.sort(gd) - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_chase_is_bitwise_equal_to_oracle(sep, handle):
+    """Same operation order, no contraction, IEEE div/sqrt: dv, ev bit for bit."""
+    n = 300
+    d, e = sep.inputs.tridiag_linear(n, seed=3)
+    (od, oe, *_), (gd, ge, *_) = _solve_both(sep, handle, d, e, 128)
+    assert np.array_equal(gd, od) and np.array_equal(ge, oe)
+
+
+def test_random_spectrum(sep, handle):
+    r = np.random.default_rng(5)
+    n = 160
+    d, e = r.standard_normal(n), r.standard_normal(n - 1)
+    (od, _, oQ, oconv, oit, _), (gd, _, gQ, conv, it, _) = _solve_both(sep, handle, d, e, 32)
+    assert conv == oconv and it == oit
+    ref = sl.eigvalsh_tridiagonal(d, e)
+    assert np.abs(np.sort(gd) - ref).max() <= 1e-12 * np.abs(ref).max()
+    A = _dense(d, e)
+    assert np.linalg.norm(A @ gQ - gQ * gd) <= 1e-13 * np.linalg.norm(A)
+
+
+def test_float32_solve(sep, handle):
+    n = 128
+    d, e = sep.inputs.tridiag_linear(n, seed=8)
+    (od, _, oQ, oconv, oit, _), (gd, _, gQ, conv, it, _) = _solve_both(
+        sep, handle, d, e, 64, dtype=np.float32)
+    assert conv and oconv and it == oit
+    ref = sl.eigvalsh_tridiagonal(d, e)
+    assert np.abs(np.sort(gd.astype(np.float64)) - ref).max() <= 1e-5 * np.abs(ref).max()
+    A = _dense(d, e)
+    assert np.linalg.norm(A @ gQ - gQ * gd) <= 1e-5 * np.linalg.norm(A)
+
+
+def test_eigenvalues_only_and_device_entry(sep, handle):
+    import torch
+    n = 220
+    d, e = sep.inputs.tridiag_linear(n, seed=2)
+    od, oe = d.copy(), e.copy()
+    oracle.qr_algorithm(od, oe, None, 128)
+    H = sep.SymTridiagonal(torch.from_numpy(d).cuda(), torch.from_numpy(e).cuda())
+    _, conv, it = sep.qr_algorithm_(H, None, handle=handle)
+    assert conv
+    assert np.abs(H.dv.cpu().numpy() - od).max() <= 1e-12 * n
+    # device Q
+    H2 = sep.SymTridiagonal(torch.from_numpy(d).cuda(), torch.from_numpy(e).cuda())
+    Q = torch.eye(n, dtype=torch.float64, device="cuda").t()  # column-major (1, n) strides
+    sep.qr_algorithm_(H2, Q, handle=handle)
+    gQ = sep.to_host_colmajor(Q)
+    A = _dense(d, e)
+    lam = H2.dv.cpu().numpy()
+    assert np.linalg.norm(A @ gQ - gQ * lam) <= 1e-13 * np.linalg.norm(A)
+
+
+def test_diagonal_input(sep, handle):
+    n = 12
+    d = np.arange(1.0, n + 1)
+    e = np.zeros(n - 1)
+    H = sep.SymTridiagonal(d.copy(), e.copy())
+    Q = np.asfortranarray(np.eye(n))
+    _, conv, it = sep.qr_algorithm_(H, Q, handle=handle)
+    assert conv and it == 0
+    assert np.array_equal(Q, np.eye(n)) and np.array_equal(H.dv, d)
+
+
+def test_toeplitz_closed_form(sep, handle):
+    n, a, b = 96, 2.0, -1.0
+    H = sep.SymTridiagonal(np.full(n, a), np.full(n - 1, b))
+    Q = np.asfortranarray(np.eye(n))
+    _, conv, _ = sep.qr_algorithm_(H, Q, handle=handle)
+    assert conv
+    j = np.arange(1, n + 1)
+    exact = np.sort(a + 2 * b * np.cos(j * np.pi / (n + 1)))
+    assert np.abs(np.sort(H.dv) - exact).max() <= 1e-13 * 4
+
+
+def test_not_converged_matches_oracle(sep, handle):
+    n = 80
+    d, e = sep.inputs.tridiag_linear(n, seed=4)
+    (od, oe, oQ, oconv, oit, _), (gd, ge, gQ, conv, it, _) = _solve_both(
+        sep, handle, d, e, 16, maxiter=10)
+    assert not oconv and not conv and it == oit == 11
+    assert np.abs(gd - od).max() <= 1e-12 * n
+    assert np.linalg.norm(gQ - oQ) <= 1e-13 * n
+
+
+def test_errors(sep, handle):
+    with pytest.raises(AssertionError):
+        sep.qr_algorithm_(sep.SymTridiagonal(np.ones(5), np.ones(4)), None, handle=handle)
+    with pytest.raises(ValueError):
+        sep.qr_algorithm_(sep.SymTridiagonal(np.ones(6), np.ones(5)), None, 0, handle=handle)
+
+
+def test_readme_example_n2000(sep, handle):
+    """Configuration 1 (the README example): n = 2000, d = 1:n, e ~ U[0,1), Q = I."""
+    n = 2000
+    d, e = sep.inputs.tridiag_linear(n)
+    (od, oe, oQ, oconv, oit, onrot), (gd, ge, gQ, conv, it, nrot) = _solve_both(
+        sep, handle, d, e, 128)
+    assert conv and it == oit and nrot == onrot
+    assert np.abs(gd - od).max() <= 1e-12 * n
+    assert np.linalg.norm(gQ - oQ) <= 1e-13 * np.sqrt(n) * 10
+    A = _dense(d, e)
+    assert np.linalg.norm(A @ gQ - gQ * gd) <= 1e-13 * np.linalg.norm(A)
+    assert np.linalg.norm(gQ.T @ gQ - np.eye(n)) <= 1e-13 * n
