@@ -48,6 +48,14 @@ const char *sep_version(void);
 int sep_set_stream(sep_handle *h, void *stream);
 int sep_synchronize(sep_handle *h);
 
+/* Kernel timing (instrumentation, no counterpart in the reference).  When enabled, every
+ * launch of the main bulk-apply kernel is bracketed by HIP events on the stream it runs
+ * on; sep_get_timing returns the summed kernel time (ms), the summed time of the
+ * coefficient-preparation kernels that precede it (ms) and the number of apply launches
+ * since the last reset.  Enabling or disabling resets the counters. */
+int sep_set_timing(sep_handle *h, int enable);
+int sep_get_timing(sep_handle *h, double *apply_ms, double *prep_ms, int64_t *launches);
+
 /* ---- Seam 2: bulk Givens application in wave order --------------------------
  * Replaces bulk_wave_order_2x2_rmul!(A, givens) and its _parallel! variant
  * (src/wave.jl:62 and :36; called from src/bulge_chasing.jl:282 and

@@ -48,7 +48,7 @@ SEP_ERR_UNSUPPORTED = -6
 # Every symbol include/sep.h declares (checked by the CPU test suite).
 EXPORTS = (
     "sep_create", "sep_destroy", "sep_last_error", "sep_version", "sep_set_stream",
-    "sep_synchronize",
+    "sep_synchronize", "sep_set_timing", "sep_get_timing",
     "sep_bulk_rmul_f64", "sep_bulk_rmul_f32", "sep_bulk_rmul_f64_dev", "sep_bulk_rmul_f32_dev",
     "sep_qr_algorithm_f64", "sep_qr_algorithm_f32", "sep_qr_algorithm_f64_dev",
     "sep_qr_algorithm_f32_dev",
@@ -94,6 +94,8 @@ def _declare(L: C.CDLL) -> None:
     L.sep_version.argtypes = []
     L.sep_set_stream.argtypes = [_P, _P]
     L.sep_synchronize.argtypes = [_P]
+    L.sep_set_timing.argtypes = [_P, C.c_int]
+    L.sep_get_timing.argtypes = [_P, _P, _P, _P]
     for suf, ct in (("f64", C.c_double), ("f32", C.c_float)):
         for dev in ("", "_dev"):
             f = getattr(L, f"sep_bulk_rmul_{suf}{dev}")
@@ -150,6 +152,16 @@ class Handle:
 
     def set_stream(self, stream_ptr: Optional[int]) -> None:
         _check(lib().sep_set_stream(self._h, stream_ptr))
+
+    def set_timing(self, enable: bool) -> None:
+        """Bracket every bulk-apply kernel with HIP events (resets the counters)."""
+        _check(lib().sep_set_timing(self._h, 1 if enable else 0))
+
+    def get_timing(self) -> Tuple[float, float, int]:
+        """(apply-kernel ms, preparation-kernels ms, apply launches) since the last reset."""
+        a, p, n = C.c_double(), C.c_double(), C.c_int64()
+        _check(lib().sep_get_timing(self._h, C.byref(a), C.byref(p), C.byref(n)))
+        return a.value, p.value, n.value
 
     def __del__(self):  # pragma: no cover - interpreter shutdown order
         try:

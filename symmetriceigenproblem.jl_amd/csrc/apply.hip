@@ -23,6 +23,8 @@
 //    order (per wave, per time step, KG contiguous pairs, identity padding outside
 //    the grid), which the main kernel reads with scalar loads into SGPRs.  The inner
 //    loop is then 4 VALU ops per rotation and row (2 mul + 2 fma) and nothing else.
+#include <type_traits>
+
 #include "sep_internal.hpp"
 
 namespace sep {
@@ -217,8 +219,8 @@ __global__ void __launch_bounds__(kLanes * G)
 }
 
 template <typename T, int KG, int G, int S>
-int run_passes(hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T* A, int64_t lda, const T* cs,
-               int64_t ld_cs, int64_t k)
+int run_passes(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T* A,
+               int64_t lda, const T* cs, int64_t ld_cs, int64_t k)
 {
     constexpr int Kp = KG * G;
     const int T_out = (int)(n + 2 * Kp - 2);
@@ -231,12 +233,16 @@ int run_passes(hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T* A, int64_t
     for (int64_t L0 = 0; L0 < k; L0 += Kp) {
         const int64_t total = (int64_t)G * U * KG;
         const int rblocks = (int)std::min<int64_t>((total + 255) / 256, 8192);
+        const int tp = timing_mark(h, st);
         hipLaunchKernelGGL((k_relayout<T, KG, G, S>), dim3(rblocks), dim3(256), 0, st, tab, cs,
                            ld_cs, (int)n, (int)k, (int)L0, U);
         SEP_CHECK_LAUNCH();
+        const int ta = timing_mark(h, st);
+        timing_pair(h, false, tp, ta);
         hipLaunchKernelGGL((k_apply_systolic<T, KG, G, S>), dim3((unsigned)rb),
                            dim3(kLanes * G), 0, st, A, lda, (int)m, (int)n, tab, U, T_out, NI);
         SEP_CHECK_LAUNCH();
+        timing_pair(h, true, ta, timing_mark(h, st));
     }
     return SEP_OK;
 }
@@ -245,7 +251,7 @@ int run_passes(hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T* A, int64_t
 
 template <typename T>
 int bulk_apply_dev(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T* A,
-                   int64_t lda, const T* cs, int64_t ld_cs, int64_t k)
+                   int64_t lda, const T* cs, int64_t ld_cs, int64_t k, bool rotations)
 {
     (void)h;
     if (m <= 0 || n <= 1 || k <= 0) return SEP_OK;
@@ -254,13 +260,17 @@ int bulk_apply_dev(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64
                   (long long)m, (long long)n);
         return SEP_ERR_ARG;
     }
-    if (k <= 32) return run_passes<T, 8, 4, 16>(st, w, m, n, A, lda, cs, ld_cs, k);
-    return run_passes<T, 8, 8, 16>(st, w, m, n, A, lda, cs, ld_cs, k);
+    if constexpr (std::is_same<T, double>::value) {
+        if (h == nullptr || !h->force_sgpr_apply)
+            return bulk_apply_dpp_f64(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    }
+    if (k <= 32) return run_passes<T, 8, 4, 16>(h, st, w, m, n, A, lda, cs, ld_cs, k);
+    return run_passes<T, 8, 8, 16>(h, st, w, m, n, A, lda, cs, ld_cs, k);
 }
 
 template int bulk_apply_dev<double>(sep_handle*, hipStream_t, ApplyWork&, int64_t, int64_t,
-                                    double*, int64_t, const double*, int64_t, int64_t);
+                                    double*, int64_t, const double*, int64_t, int64_t, bool);
 template int bulk_apply_dev<float>(sep_handle*, hipStream_t, ApplyWork&, int64_t, int64_t, float*,
-                                   int64_t, const float*, int64_t, int64_t);
+                                   int64_t, const float*, int64_t, int64_t, bool);
 
 }  // namespace sep

@@ -1,6 +1,7 @@
 // capi.hip -- the extern "C" surface of libsep_amd (declared in include/sep.h):
 // handle lifetime, error reporting, argument validation and host<->device staging.
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 
 #include "sep_internal.hpp"
@@ -34,6 +35,19 @@ int DevBuf::reserve(size_t want)
     }
     bytes = want;
     return SEP_OK;
+}
+
+int timing_mark(sep_handle* h, hipStream_t st)
+{
+    if (!h || !h->timing) return -1;
+    if (h->tev_used == (int)h->tev.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        h->tev.push_back(e);
+    }
+    const int i = h->tev_used++;
+    if (hipEventRecord(h->tev[i], st) != hipSuccess) return -1;
+    return i;
 }
 
 void DevBuf::release()
@@ -224,6 +238,7 @@ int sep_create(sep_handle** out, int device)
     }
     sep_handle* h = new sep_handle();
     h->device = device;
+    if (const char* ap = std::getenv("SEP_APPLY")) h->force_sgpr_apply = std::strcmp(ap, "sgpr") == 0;
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->chase_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_a, hipEventDisableTiming);
@@ -248,7 +263,10 @@ int sep_destroy(sep_handle* h)
     h->host_io_cs.release();
     h->host_io_d.release();
     h->host_io_e.release();
-    for (auto& w : h->apply_work) w.table.release();
+    for (auto& w : h->apply_work) {
+        w.table.release();
+        w.aux.release();
+    }
     for (auto& g : h->grid) g.release();
     h->state.release();
     h->counters.release();
@@ -256,11 +274,52 @@ int sep_destroy(sep_handle* h)
     h->batch_meta.release();
     h->batch_scratch.release();
     h->bwd_W.release();
+    for (auto e : h->tev) (void)hipEventDestroy(e);
     if (h->ev_a) (void)hipEventDestroy(h->ev_a);
     if (h->ev_b) (void)hipEventDestroy(h->ev_b);
     if (h->chase_stream) (void)hipStreamDestroy(h->chase_stream);
     if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
+    return SEP_OK;
+}
+
+int sep_set_timing(sep_handle* h, int enable)
+{
+    if (!h) {
+        set_error("sep_set_timing: null handle");
+        return SEP_ERR_ARG;
+    }
+    SEP_HIP(hipSetDevice(h->device));
+    SEP_HIP(hipStreamSynchronize(h->stream));
+    h->timing = enable != 0;
+    h->tapply.clear();
+    h->tprep.clear();
+    h->tev_used = 0;
+    return SEP_OK;
+}
+
+int sep_get_timing(sep_handle* h, double* apply_ms, double* prep_ms, int64_t* launches)
+{
+    if (!h) {
+        set_error("sep_get_timing: null handle");
+        return SEP_ERR_ARG;
+    }
+    SEP_HIP(hipSetDevice(h->device));
+    SEP_HIP(hipStreamSynchronize(h->stream));
+    double a = 0, p = 0;
+    for (auto& pr : h->tapply) {
+        float ms = 0;
+        SEP_HIP(hipEventElapsedTime(&ms, h->tev[pr.first], h->tev[pr.second]));
+        a += ms;
+    }
+    for (auto& pr : h->tprep) {
+        float ms = 0;
+        SEP_HIP(hipEventElapsedTime(&ms, h->tev[pr.first], h->tev[pr.second]));
+        p += ms;
+    }
+    if (apply_ms) *apply_ms = a;
+    if (prep_ms) *prep_ms = p;
+    if (launches) *launches = (int64_t)h->tapply.size();
     return SEP_OK;
 }
 

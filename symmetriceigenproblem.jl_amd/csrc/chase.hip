@@ -134,7 +134,8 @@ int qr_algorithm_dev(sep_handle* h, int64_t n64, T* dv, T* ev, T* Q, int64_t ldq
             if (hipStreamWaitEvent(as, chased, 0) != hipSuccess) { rc = SEP_ERR_HIP; break; }
             // rotations ilo..ihi (1-based) touch columns ilo-1 .. ihi (0-based)
             rc = bulk_apply_dev<T>(h, as, h->apply_work[cur], n, ihi - ilo + 2,
-                                   Q + (ilo - 1) * ldq, ldq, grid + 2 * (ilo - 1), n - 1, depth);
+                                   Q + (ilo - 1) * ldq, ldq, grid + 2 * (ilo - 1), n - 1, depth,
+                                   /*rotations=*/true);
             if (rc) break;
             if (hipEventRecord(applied[cur], as) != hipSuccess) { rc = SEP_ERR_HIP; break; }
             applied_pending[cur] = true;

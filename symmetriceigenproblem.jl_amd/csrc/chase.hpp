@@ -59,14 +59,23 @@ __device__ __forceinline__ float tabs(float x) { return __builtin_fabsf(x); }
 
 #pragma clang fp contract(off)
 
+template <typename T>
+struct Rot {
+    T c, s, r;
+};
+
 // LinearAlgebra.givensAlgorithm (classic DLARTG), via get_rotation
 // (src/simple_givens.jl:20-23).  g == 0 -> (1,0,f); f == 0 -> (0,1,g); otherwise
 // c = f/r, s = g/r, r = sqrt(f^2+g^2), negated when |f| > |g| and c < 0, with
-// power-of-two rescaling when max(|f|,|g|) leaves [2^-485, 2^485] (Float64).
+// power-of-two rescaling when max(|f|,|g|) leaves (2^-485, 2^485) (Float64).
+// Rescaled path; reached only for extreme magnitudes.
 template <typename T>
-__device__ __attribute__((noinline)) void givens_rescaled(T f, T g, T& c, T& s, T& r)
+__device__ __attribute__((noinline)) Rot<T> givens_rescaled(T f, T g)
 {
     const T safmn2 = Consts<T>::safmn2, safmx2 = Consts<T>::safmx2;
+    Rot<T> o;
+    if (g == T(0)) return {T(1), T(0), f};
+    if (f == T(0)) return {T(0), T(1), g};
     T f1 = f, g1 = g;
     T scale = tabs(f1) > tabs(g1) ? tabs(f1) : tabs(g1);
     int count = 0;
@@ -77,10 +86,10 @@ __device__ __attribute__((noinline)) void givens_rescaled(T f, T g, T& c, T& s, 
             g1 *= safmn2;
             scale = tabs(f1) > tabs(g1) ? tabs(f1) : tabs(g1);
         } while (scale >= safmx2 && count < 20);
-        r = sqrt(f1 * f1 + g1 * g1);
-        c = f1 / r;
-        s = g1 / r;
-        for (int i = 0; i < count; i++) r *= safmx2;
+        o.r = sqrt(f1 * f1 + g1 * g1);
+        o.c = f1 / o.r;
+        o.s = g1 / o.r;
+        for (int i = 0; i < count; i++) o.r *= safmx2;
     } else {
         do {
             count++;
@@ -88,41 +97,37 @@ __device__ __attribute__((noinline)) void givens_rescaled(T f, T g, T& c, T& s, 
             g1 *= safmx2;
             scale = tabs(f1) > tabs(g1) ? tabs(f1) : tabs(g1);
         } while (scale <= safmn2);
-        r = sqrt(f1 * f1 + g1 * g1);
-        c = f1 / r;
-        s = g1 / r;
-        for (int i = 0; i < count; i++) r *= safmn2;
+        o.r = sqrt(f1 * f1 + g1 * g1);
+        o.c = f1 / o.r;
+        o.s = g1 / o.r;
+        for (int i = 0; i < count; i++) o.r *= safmn2;
     }
+    if (tabs(f) > tabs(g) && o.c < T(0)) {
+        o.c = -o.c;
+        o.s = -o.s;
+        o.r = -o.r;
+    }
+    return o;
 }
 
+// All lanes of the wave evaluate the chase with identical values; the rare-path test is
+// made wave-uniform (readfirstlane) so it compiles to a scalar branch, and the common
+// path uses selects only.
 template <typename T>
-__device__ __forceinline__ void givens(T f, T g, T& c, T& s, T& r)
+__device__ __forceinline__ Rot<T> givens(T f, T g)
 {
     const T af = tabs(f), ag = tabs(g);
     const T scale = af > ag ? af : ag;
-    if (scale >= Consts<T>::safmx2 || scale <= Consts<T>::safmn2) {
-        if (g == T(0)) {
-            c = T(1); s = T(0); r = f;
-            return;
-        }
-        if (f == T(0)) {
-            c = T(0); s = T(1); r = g;
-            return;
-        }
-        givens_rescaled(f, g, c, s, r);
-    } else {
-        r = sqrt(f * f + g * g);
-        c = f / r;
-        s = g / r;
-        if (g == T(0)) {  // also covers f == g == 0 (then scale <= safmn2 anyway)
-            c = T(1); s = T(0); r = f;
-        } else if (f == T(0)) {
-            c = T(0); s = T(1); r = g;
-        }
-    }
-    if (af > ag && c < T(0)) {
-        c = -c; s = -s; r = -r;
-    }
+    const int rare = (scale >= Consts<T>::safmx2) | (scale <= Consts<T>::safmn2);
+    if (__builtin_expect(__builtin_amdgcn_readfirstlane(rare), 0)) return givens_rescaled(f, g);
+    const T r0 = sqrt(f * f + g * g);
+    T c = f / r0, s = g / r0, r = r0;
+    const bool g0 = g == T(0), f0 = f == T(0);
+    c = g0 ? T(1) : (f0 ? T(0) : c);
+    s = g0 ? T(0) : (f0 ? T(1) : s);
+    r = g0 ? f : (f0 ? g : r);
+    const bool flip = (af > ag) & (c < T(0));
+    return {flip ? -c : c, flip ? -s : s, flip ? -r : r};
 }
 
 // is_offdiagonal_small, src/bulge_chasing.jl:7-8 (1-based i; dv, ev 0-based arrays)
@@ -202,13 +207,12 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
     int* __restrict__ lt = P.lt;
     T* __restrict__ grid = P.grid;
     const int64_t ldg = P.ldg;
-    T c, s, nrm;
-
     // ---- first rotation, :17-63
     const T H11 = dv[from - 1];
     const T H21 = ev[from - 1];
     const T H22 = dv[from];
-    givens(H11 - mu, H21, c, s, nrm);
+    const Rot<T> G1 = givens(H11 - mu, H21);
+    T c = G1.c, s = G1.s;
     const T H11p = c * H11 + s * H21;
     const T H21p = -s * H11 + c * H21;
     const T H12p = c * H21 + s * H22;
@@ -250,7 +254,8 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                 nL = lt[i - 1];
             }
         }
-        for (int i0 = i_first; i0 <= i_last; i0 += kWave) {
+        for (int i0u = i_first; i0u <= i_last; i0u += kWave) {
+            const int i0 = __builtin_amdgcn_readfirstlane(i0u);
             const T D = nD, E = nE;
             const int Lt = nL;
             {  // prefetch next chunk
@@ -264,13 +269,16 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                     nL = lt[i - 1];
                 }
             }
-            const int cnt = min(kWave, i_last - i0 + 1);
+            const int cnt = __builtin_amdgcn_readfirstlane(min(kWave, i_last - i0 + 1));
             T oN = T(0), oD = T(0), oC = T(0), oS = T(0);
             for (int j = 0; j < cnt; ++j) {
                 const bool mine = lane == j;  // lane j keeps step i0 + j's outputs
                 const int i = i0 + j;
                 const T B22 = rdlane(D, j);
-                givens(p1, bulge, c, s, nrm);
+                const Rot<T> G = givens(p1, bulge);
+                c = G.c;
+                s = G.s;
+                const T nrm = G.r;
                 const T B11 = d_cur, B21 = e_cur;
                 const T B11p = c * B11 + s * B21;
                 const T B21p = -s * B11 + c * B21;
@@ -343,7 +351,8 @@ __device__ void chase_block(const Problem<T>& P, T tol, int64_t maxiter, int64_t
             break;
         }
         int from = to;
-        if (from > start) from = deflation_scan(P, to, start, tol, lane) + 1;
+        if (from > start)
+            from = __builtin_amdgcn_readfirstlane(deflation_scan(P, to, start, tol, lane) + 1);
         if (to <= start) break;
         if (from == to) {
             if (lane == 0) P.ev[from - 2] = T(0);
@@ -351,7 +360,7 @@ __device__ void chase_block(const Problem<T>& P, T tol, int64_t maxiter, int64_t
             if (to == lbea) lbea -= 1;
             to -= 1;
         } else {
-            const int thick = max_thickness(P, from, to, lane);
+            const int thick = __builtin_amdgcn_readfirstlane(max_thickness(P, from, to, lane));
             if (thick == P.max_layers) {
                 to = from - 1;
             } else {

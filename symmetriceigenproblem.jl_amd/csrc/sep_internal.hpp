@@ -57,6 +57,7 @@ struct ApplyGeom {
 // Per-launch workspace of the systolic apply: the relaid (c,s) sweep table.
 struct ApplyWork {
     DevBuf table;
+    DevBuf aux;  // column sign masks, output signs, normalisation flag
 };
 
 // Batched-solver tuning entry (chosen by the autotuner).
@@ -86,14 +87,33 @@ struct sep_handle {
     int batched_choice = -1;
     std::vector<int64_t> tune_key;
     int tune_choice = -1;
+    bool force_sgpr_apply = false;  // SEP_APPLY=sgpr selects the scalar-fed Float64 kernel
+    // kernel timing (sep_set_timing): event pairs recorded around apply / prep kernels
+    bool timing = false;
+    std::vector<hipEvent_t> tev;            // pool, reused
+    std::vector<std::pair<int, int>> tapply, tprep;  // indices into tev
+    int tev_used = 0;
 };
 
 namespace sep {
+// Record a timing event on `st` (returns its index, or -1 when timing is off).
+int timing_mark(sep_handle* h, hipStream_t st);
+inline void timing_pair(sep_handle* h, bool apply, int a, int b)
+{
+    if (a >= 0 && b >= 0) (apply ? h->tapply : h->tprep).emplace_back(a, b);
+}
+}  // namespace sep
 
-// apply.hip
+namespace sep {
+
+// apply.hip.  `rotations` promises c^2 + s^2 = 1 for every pair (skips the check).
 template <typename T>
 int bulk_apply_dev(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T* A,
-                   int64_t lda, const T* cs, int64_t ld_cs, int64_t k);
+                   int64_t lda, const T* cs, int64_t ld_cs, int64_t k, bool rotations = false);
+// apply_dpp.hip (Float64 DPP-fed kernels)
+int bulk_apply_dpp_f64(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n,
+                       double* A, int64_t lda, const double* cs, int64_t ld_cs, int64_t k,
+                       bool rotations);
 
 // chase.hip
 template <typename T>
