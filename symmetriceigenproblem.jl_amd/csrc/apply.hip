@@ -195,7 +195,7 @@ __device__ __forceinline__ void systolic_wave(T* __restrict__ A, int64_t lda, in
                 systolic_interval<T, KG, G, S, FIRST, LAST, true>(Ar, lda, n, tp, win, pf,
                                                                   lds_in, lds_out, t0, g);
         }
-        __syncthreads();
+        lds_barrier();
     }
 }
 

@@ -238,7 +238,10 @@ int sep_create(sep_handle** out, int device)
     }
     sep_handle* h = new sep_handle();
     h->device = device;
-    if (const char* ap = std::getenv("SEP_APPLY")) h->force_sgpr_apply = std::strcmp(ap, "sgpr") == 0;
+    if (const char* ap = std::getenv("SEP_APPLY")) {
+        h->force_sgpr_apply = std::strcmp(ap, "sgpr") == 0;
+        h->apply_geom = std::strcmp(ap, "deep") == 0 ? 1 : 0;
+    }
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->chase_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_a, hipEventDisableTiming);

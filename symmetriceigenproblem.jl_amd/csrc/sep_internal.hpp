@@ -37,6 +37,18 @@ void set_error(const char* fmt, ...);
         }                                                                               \
     } while (0)
 
+// Workgroup barrier that orders LDS only.  __syncthreads() also waits for every
+// outstanding global load and store of the wave (vmcnt(0)), which would drain the
+// multi-interval prefetches of the apply kernels at each interval; their waves exchange
+// data through LDS only (each column of Q is read by the first wave strictly before the
+// last wave writes it), so an LDS fence is all that is needed.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // A growable device buffer owned by the handle.
 struct DevBuf {
     void* p = nullptr;
@@ -88,6 +100,7 @@ struct sep_handle {
     std::vector<int64_t> tune_key;
     int tune_choice = -1;
     bool force_sgpr_apply = false;  // SEP_APPLY=sgpr selects the scalar-fed Float64 kernel
+    int apply_geom = 0;             // SEP_APPLY=deep selects the 16-wave, 4-layer geometry
     // kernel timing (sep_set_timing): event pairs recorded around apply / prep kernels
     bool timing = false;
     std::vector<hipEvent_t> tev;            // pool, reused
