@@ -41,6 +41,7 @@
 #define SUF _f64
 #define FABS fabs
 #define SQRT sqrt
+#define FMA fma
 #define SAFMN2 0x1p-485
 #define VEC_W 8
 #define orc_givens PFX(orc_givens)
@@ -57,6 +58,7 @@
 #undef SUF
 #undef FABS
 #undef SQRT
+#undef FMA
 #undef SAFMN2
 #undef VEC_W
 
@@ -65,6 +67,7 @@
 #define SUF _f32
 #define FABS fabsf
 #define SQRT sqrtf
+#define FMA fmaf
 #define SAFMN2 0x1p-51f
 #define VEC_W 16
 #include "sep_oracle_impl.h"
@@ -72,6 +75,7 @@
 #undef SUF
 #undef FABS
 #undef SQRT
+#undef FMA
 #undef SAFMN2
 #undef VEC_W
 

@@ -115,13 +115,15 @@ void FN(orc_single_shift)(int64_t n, T *dv, T *ev, int64_t from, int64_t to, T m
     FN(orc_givens)(p1, p2, &c, &s, &nrm);
     /* :27-47 */
     T H22 = DV(from + 1);
-    T H11p = c * H11 + s * H21;
-    T H21p = -s * H11 + c * H21;
-    T H12p = c * H21 + s * H22;
-    T H22p = -s * H21 + c * H22;
-    T H11pp = H11p * c + H12p * s;
-    T H12pp = H11p * -s + H12p * c;
-    T H22pp = H21p * -s + H22p * c;
+    /* x*y + z*w is evaluated as fma(x, y, z*w): single_shift! is @fastmath, which lets
+     * the compiler contract; the device chase uses the identical contraction. */
+    T H11p = FMA(c, H11, s * H21);
+    T H21p = FMA(-s, H11, c * H21);
+    T H12p = FMA(c, H21, s * H22);
+    T H22p = FMA(-s, H21, c * H22);
+    T H11pp = FMA(H11p, c, H12p * s);
+    T H12pp = FMA(H11p, -s, H12p * c);
+    T H22pp = FMA(H21p, -s, H22p * c);
     DV(from + 0) = H11pp;
     DV(from + 1) = H22pp;
     EV(from + 0) = H12pp;
@@ -148,13 +150,13 @@ void FN(orc_single_shift)(int64_t n, T *dv, T *ev, int64_t from, int64_t to, T m
         T B11 = DV(i + 0);
         T B22 = DV(i + 1);
         T B21 = EV(i + 0);
-        T B11p = c * B11 + s * B21;
-        T B21p = -s * B11 + c * B21;
-        T B12p = c * B21 + s * B22;
-        T B22p = -s * B21 + c * B22;
-        T B11pp = B11p * c + B12p * s;
-        T B12pp = B11p * -s + B12p * c;
-        T B22pp = B21p * -s + B22p * c;
+        T B11p = FMA(c, B11, s * B21);
+        T B21p = FMA(-s, B11, c * B21);
+        T B12p = FMA(c, B21, s * B22);
+        T B22p = FMA(-s, B21, c * B22);
+        T B11pp = FMA(B11p, c, B12p * s);
+        T B12pp = FMA(B11p, -s, B12p * c);
+        T B22pp = FMA(B21p, -s, B22p * c);
         DV(i + 0) = B11pp;
         DV(i + 1) = B22pp;
         EV(i + 0) = B12pp;

@@ -55,6 +55,16 @@ __device__ __forceinline__ float rdlane(float v, int j)
 __device__ __forceinline__ int rdlane(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
 
 __device__ __forceinline__ double tabs(double x) { return __builtin_fabs(x); }
+
+// One (c, s) pair as a single 16-byte (8-byte) store.
+__device__ __forceinline__ void store_pair(double* p, double c, double s)
+{
+    *reinterpret_cast<double2*>(p) = make_double2(c, s);
+}
+__device__ __forceinline__ void store_pair(float* p, float c, float s)
+{
+    *reinterpret_cast<float2*>(p) = make_float2(c, s);
+}
 __device__ __forceinline__ float tabs(float x) { return __builtin_fabsf(x); }
 
 #pragma clang fp contract(off)
@@ -68,9 +78,9 @@ struct Rot {
 // (src/simple_givens.jl:20-23).  g == 0 -> (1,0,f); f == 0 -> (0,1,g); otherwise
 // c = f/r, s = g/r, r = sqrt(f^2+g^2), negated when |f| > |g| and c < 0, with
 // power-of-two rescaling when max(|f|,|g|) leaves (2^-485, 2^485) (Float64).
-// Rescaled path; reached only for extreme magnitudes.
+// This complete version serves the rare cases of givens() below.
 template <typename T>
-__device__ __attribute__((noinline)) Rot<T> givens_rescaled(T f, T g)
+__device__ __attribute__((noinline)) Rot<T> givens_slow(T f, T g)
 {
     const T safmn2 = Consts<T>::safmn2, safmx2 = Consts<T>::safmx2;
     Rot<T> o;
@@ -90,7 +100,7 @@ __device__ __attribute__((noinline)) Rot<T> givens_rescaled(T f, T g)
         o.c = f1 / o.r;
         o.s = g1 / o.r;
         for (int i = 0; i < count; i++) o.r *= safmx2;
-    } else {
+    } else if (scale <= safmn2) {
         do {
             count++;
             f1 *= safmx2;
@@ -101,6 +111,10 @@ __device__ __attribute__((noinline)) Rot<T> givens_rescaled(T f, T g)
         o.c = f1 / o.r;
         o.s = g1 / o.r;
         for (int i = 0; i < count; i++) o.r *= safmn2;
+    } else {
+        o.r = sqrt(f1 * f1 + g1 * g1);
+        o.c = f1 / o.r;
+        o.s = g1 / o.r;
     }
     if (tabs(f) > tabs(g) && o.c < T(0)) {
         o.c = -o.c;
@@ -110,24 +124,79 @@ __device__ __attribute__((noinline)) Rot<T> givens_rescaled(T f, T g)
     return o;
 }
 
-// All lanes of the wave evaluate the chase with identical values; the rare-path test is
-// made wave-uniform (readfirstlane) so it compiles to a scalar branch, and the common
-// path uses selects only.
+// Correctly rounded square root for arguments the common path of givens() can produce
+// (t = f^2 + g^2 with 2^-380 < max(|f|, |g|) < 2^485): the same reciprocal-square-root
+// refinement the compiler emits for sqrt(double), without its rescaling of tiny
+// arguments, which never applies in that range -- so the result is the same bits.
+__device__ __forceinline__ double sqrt_common(double t)
+{
+    const double y = __builtin_amdgcn_rsq(t);
+    double gq = t * y, h = 0.5 * y;
+    const double r = __builtin_fma(-h, gq, 0.5);
+    gq = __builtin_fma(gq, r, gq);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-gq, gq, t);
+    gq = __builtin_fma(d, h, gq);
+    d = __builtin_fma(-gq, gq, t);
+    gq = __builtin_fma(d, h, gq);
+    return gq;
+}
+__device__ __forceinline__ float sqrt_common(float t) { return sqrtf(t); }
+
+template <typename T>
+struct CommonRange;
+template <>
+struct CommonRange<double> {
+    static constexpr double lo = 0x1p-380;  // smallest |f|, |g| of the common path
+};
+template <>
+struct CommonRange<float> {
+    static constexpr float lo = 0x1p-51f;
+};
+
+// Two IEEE quotients over the same denominator r (2^-380 <= |num| <= r < 2^486).  This is
+// the compiler's double-division sequence (reciprocal, two Newton steps, quotient,
+// remainder, final fma) with its operand scaling (v_div_scale / v_div_fmas /
+// v_div_fixup) dropped: for operands and quotients this far from the exponent limits the
+// scaling steps are identities, so the results are the same correctly rounded bits, and
+// the reciprocal refinement is shared by both quotients.
+__device__ __forceinline__ void div2_common(double a, double b, double r, double& qa, double& qb)
+{
+    double y = __builtin_amdgcn_rcp(r);
+    double e = __builtin_fma(-r, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-r, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    const double q1 = a * y, q2 = b * y;
+    const double r1 = __builtin_fma(-r, q1, a), r2 = __builtin_fma(-r, q2, b);
+    qa = __builtin_fma(r1, y, q1);
+    qb = __builtin_fma(r2, y, q2);
+}
+__device__ __forceinline__ void div2_common(float a, float b, float r, float& qa, float& qb)
+{
+    qa = a / r;
+    qb = b / r;
+}
+
+// All lanes of the wave evaluate the chase with identical values.  The common path
+// (2^-380 < |f|, |g| < 2^485) is computed unconditionally: the sign rule is applied to
+// the numerators (-(f/r) == (-f)/r exactly), and the wave-uniform rare-case test only
+// afterwards redirects to givens_slow, so its branch never stalls the path.
 template <typename T>
 __device__ __forceinline__ Rot<T> givens(T f, T g)
 {
     const T af = tabs(f), ag = tabs(g);
-    const T scale = af > ag ? af : ag;
-    const int rare = (scale >= Consts<T>::safmx2) | (scale <= Consts<T>::safmn2);
-    if (__builtin_expect(__builtin_amdgcn_readfirstlane(rare), 0)) return givens_rescaled(f, g);
-    const T r0 = sqrt(f * f + g * g);
-    T c = f / r0, s = g / r0, r = r0;
-    const bool g0 = g == T(0), f0 = f == T(0);
-    c = g0 ? T(1) : (f0 ? T(0) : c);
-    s = g0 ? T(0) : (f0 ? T(1) : s);
-    r = g0 ? f : (f0 ? g : r);
-    const bool flip = (af > ag) & (c < T(0));
-    return {flip ? -c : c, flip ? -s : s, flip ? -r : r};
+    const T big = __builtin_fmax(af, ag), small = __builtin_fmin(af, ag);
+    const bool rare = !(big < Consts<T>::safmx2) | !(small > CommonRange<T>::lo);
+    const bool flip = (af > ag) & (f < T(0));
+    const T fs = flip ? -f : f, gs = flip ? -g : g;
+    const T r = sqrt_common(f * f + g * g);
+    Rot<T> o;
+    div2_common(fs, gs, r, o.c, o.s);
+    o.r = flip ? -r : r;
+    asm volatile("" ::"v"(o.c), "v"(o.s), "v"(o.r));
+    if (__builtin_expect(__builtin_amdgcn_readfirstlane((int)rare), 0)) o = givens_slow(f, g);
+    return o;
 }
 
 // is_offdiagonal_small, src/bulge_chasing.jl:7-8 (1-based i; dv, ev 0-based arrays)
@@ -207,34 +276,33 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
     int* __restrict__ lt = P.lt;
     T* __restrict__ grid = P.grid;
     const int64_t ldg = P.ldg;
-    // ---- first rotation, :17-63
+    // ---- first rotation, :17-63.  x*y + z*w is evaluated as fma(x, y, z*w) (the
+    // reference's single_shift! is @fastmath; the oracle contracts identically).
     const T H11 = dv[from - 1];
     const T H21 = ev[from - 1];
     const T H22 = dv[from];
     const Rot<T> G1 = givens(H11 - mu, H21);
-    T c = G1.c, s = G1.s;
-    const T H11p = c * H11 + s * H21;
-    const T H21p = -s * H11 + c * H21;
-    const T H12p = c * H21 + s * H22;
-    const T H22p = -s * H21 + c * H22;
-    const T H11pp = H11p * c + H12p * s;
-    const T H12pp = H11p * -s + H12p * c;
-    const T H22pp = H21p * -s + H22p * c;
+    const T c1 = G1.c, s1 = G1.s;
+    const T H11p = __builtin_fma(c1, H11, s1 * H21);
+    const T H21p = __builtin_fma(-s1, H11, c1 * H21);
+    const T H12p = __builtin_fma(c1, H21, s1 * H22);
+    const T H22p = __builtin_fma(-s1, H21, c1 * H22);
+    const T H11pp = __builtin_fma(H11p, c1, H12p * s1);
+    const T H12pp = __builtin_fma(H11p, -s1, H12p * c1);
+    const T H22pp = __builtin_fma(H21p, -s1, H22p * c1);
     T bulge = T(0);
     T e_cur = T(0);  // ev[from+1] after the first step
     if (from + 2 <= to) {
         const T H23 = ev[from];
-        bulge = s * H23;
-        e_cur = c * H23;
+        bulge = s1 * H23;
+        e_cur = c1 * H23;
     }
     {
-        const int layer = lt[from - 1] + 1;
-        if (lane == 0) {
-            dv[from - 1] = H11pp;
-            lt[from - 1] = layer;
-            grid[2 * ((int64_t)(from - 1) + ldg * (layer - 1))] = c;
-            grid[2 * ((int64_t)(from - 1) + ldg * (layer - 1)) + 1] = s;
-        }
+        // every lane stores the same value to the same address (no predication)
+        const int layer = __builtin_amdgcn_readfirstlane(lt[from - 1]) + 1;
+        dv[from - 1] = H11pp;
+        lt[from - 1] = layer;
+        store_pair(grid + 2 * ((int64_t)(from - 1) + ldg * (layer - 1)), c1, s1);
     }
     // carried state entering step i: p1 = ev[i-1], d_cur = dv[i], e_cur = ev[i]
     T p1 = H12pp;
@@ -270,27 +338,25 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                 }
             }
             const int cnt = __builtin_amdgcn_readfirstlane(min(kWave, i_last - i0 + 1));
-            T oN = T(0), oD = T(0), oC = T(0), oS = T(0);
             for (int j = 0; j < cnt; ++j) {
-                const bool mine = lane == j;  // lane j keeps step i0 + j's outputs
                 const int i = i0 + j;
                 const T B22 = rdlane(D, j);
                 const Rot<T> G = givens(p1, bulge);
-                c = G.c;
-                s = G.s;
-                const T nrm = G.r;
+                const T c = G.c, s = G.s;
                 const T B11 = d_cur, B21 = e_cur;
-                const T B11p = c * B11 + s * B21;
-                const T B21p = -s * B11 + c * B21;
-                const T B12p = c * B21 + s * B22;
-                const T B22p = -s * B21 + c * B22;
-                const T B11pp = B11p * c + B12p * s;
-                const T B12pp = B11p * -s + B12p * c;
-                const T B22pp = B21p * -s + B22p * c;
-                oN = mine ? nrm : oN;
-                oD = mine ? B11pp : oD;
-                oC = mine ? c : oC;
-                oS = mine ? s : oS;
+                const T B11p = __builtin_fma(c, B11, s * B21);
+                const T B21p = __builtin_fma(-s, B11, c * B21);
+                const T B12p = __builtin_fma(c, B21, s * B22);
+                const T B22p = __builtin_fma(-s, B21, c * B22);
+                const T B11pp = __builtin_fma(B11p, c, B12p * s);
+                const T B12pp = __builtin_fma(B11p, -s, B12p * c);
+                const T B22pp = __builtin_fma(B21p, -s, B22p * c);
+                const int layer = rdlane(Lt, j) + 1;
+                // uniform stores of the finished outputs of step i
+                ev[i - 2] = G.r;   // ev[i-1] (1-based) = nrm
+                dv[i - 1] = B11pp; // dv[i]   (1-based) = B11''
+                lt[i - 1] = layer;
+                store_pair(grid + 2 * ((int64_t)(i - 1) + ldg * (layer - 1)), c, s);
                 p1 = B12pp;
                 d_cur = B22pp;
                 if (i + 2 <= to) {
@@ -299,22 +365,11 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                     e_cur = c * B23;
                 }
             }
-            if (lane < cnt) {
-                const int i = i0 + lane;
-                ev[i - 2] = oN;  // ev[i-1] (1-based) = nrm
-                dv[i - 1] = oD;  // dv[i]   (1-based) = B11''
-                const int layer = Lt + 1;
-                lt[i - 1] = layer;
-                grid[2 * ((int64_t)(i - 1) + ldg * (layer - 1))] = oC;
-                grid[2 * ((int64_t)(i - 1) + ldg * (layer - 1)) + 1] = oS;
-            }
         }
     }
     // final carried values: dv[to] and ev[to-1] (1-based); ev[to] untouched
-    if (lane == 0) {
-        dv[to - 1] = d_cur;
-        ev[to - 2] = p1;
-    }
+    dv[to - 1] = d_cur;
+    ev[to - 2] = p1;
 }
 
 // Block state exchanged with the host (int64 words).
