@@ -44,7 +44,9 @@ int sep_destroy(sep_handle *h);
 const char *sep_last_error(void);
 /* Library / kernel identification string, e.g. "libsep_amd gfx950 ...". */
 const char *sep_version(void);
-/* Run subsequent *_dev work on `stream` (a hipStream_t; NULL = the handle's own). */
+/* Run subsequent *_dev work on `stream` (a hipStream_t; NULL = the handle's own).  The
+ * handle's own streams are blocking streams, ordered with the legacy default stream; a
+ * caller that produces its device arrays on another stream passes that stream here. */
 int sep_set_stream(sep_handle *h, void *stream);
 int sep_synchronize(sep_handle *h);
 

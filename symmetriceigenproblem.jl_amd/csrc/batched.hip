@@ -1,19 +1,504 @@
-// batched.hip -- placeholder until the persistent batched solver lands.
+// batched.hip -- thousands of independent qr_algorithm! solves (Float64) in one persistent
+// launch (no counterpart in the reference, which solves one matrix per call).
+//
+// Each wavefront of the persistent grid repeatedly takes the next problem from an atomic
+// work queue (problems ordered largest first) and solves it completely:
+//   * the chase of one block of sweeps is the single-solve device chase
+//     (src/bulge_chasing.jl:166-276 semantics, same code), writing its rotations into a
+//     time-major table in the wave's scratch slot: rotation (i, L) at row i + 2L, so that
+//     the rotations of one wave-order time step are contiguous;
+//   * the table is converted in place to shear coefficients (p, q) = (s/(1+c), -s)
+//     (never-written entries stay (0, 0), the identity);
+//   * the wave applies the block to its problem's Q in wave order, KF layers per pass over
+//     Q, one row of Q per lane, the window of 2 KF columns in VGPRs, three DPP-fed FMAs per
+//     rotation and row, and Q columns and coefficient rows prefetched P steps ahead;
+//   * the outer loop (:166-285) repeats until no sweep runs or maxiter is exceeded.
+// Rows-per-lane is fixed at one (a row group of 64 rows per pass); fused depth KF, the
+// sweep block length (max_layers) and the number of persistent waves are chosen by the
+// autotuner.  The reset of the queue and the persistent kernel are captured in a HIP graph
+// that is replayed for an identical batch layout.
+#include <algorithm>
+#include <numeric>
+#include <type_traits>
+
+#include "chase.hpp"
 #include "sep_internal.hpp"
 
 namespace sep {
+namespace {
 
-int qr_batched_f64(sep_handle*, int64_t, const int64_t*, const int64_t*, const int64_t*, double*,
-                   double*, double*, int64_t, int32_t*, int64_t*)
+constexpr int kLanes = 64;
+constexpr int kPF = 8;  // prefetch distance (time steps)
+
+#define SEP_BDPP(a, c, b, lane) \
+    "v_fmac_f64_dpp " a ", " c ", " b " row_newbcast:" #lane " row_mask:0xf bank_mask:0xf\n"
+
+// Eight rotations in shear form: a += p b; b += q a; a += p b with (p_l, q_l) broadcast from
+// lanes 2l, 2l+1 of each 16-lane row of cf.
+__device__ __forceinline__ void shear8(double& a0, double& b0, double& a1, double& b1, double& a2,
+                                       double& b2, double& a3, double& b3, double& a4,
+                                       double& b4, double& a5, double& b5, double& a6,
+                                       double& b6, double& a7, double& b7, double cf)
 {
-    set_error("sep_qr_batched_f64_dev: not built yet");
-    return SEP_ERR_UNSUPPORTED;
+    asm("s_nop 1\n"
+        SEP_BDPP("%0", "%16", "%1", 0) SEP_BDPP("%2", "%16", "%3", 2)
+        SEP_BDPP("%4", "%16", "%5", 4) SEP_BDPP("%6", "%16", "%7", 6)
+        SEP_BDPP("%8", "%16", "%9", 8) SEP_BDPP("%10", "%16", "%11", 10)
+        SEP_BDPP("%12", "%16", "%13", 12) SEP_BDPP("%14", "%16", "%15", 14)
+        SEP_BDPP("%1", "%16", "%0", 1) SEP_BDPP("%3", "%16", "%2", 3)
+        SEP_BDPP("%5", "%16", "%4", 5) SEP_BDPP("%7", "%16", "%6", 7)
+        SEP_BDPP("%9", "%16", "%8", 9) SEP_BDPP("%11", "%16", "%10", 11)
+        SEP_BDPP("%13", "%16", "%12", 13) SEP_BDPP("%15", "%16", "%14", 15)
+        SEP_BDPP("%0", "%16", "%1", 0) SEP_BDPP("%2", "%16", "%3", 2)
+        SEP_BDPP("%4", "%16", "%5", 4) SEP_BDPP("%6", "%16", "%7", 6)
+        SEP_BDPP("%8", "%16", "%9", 8) SEP_BDPP("%10", "%16", "%11", 10)
+        SEP_BDPP("%12", "%16", "%13", 12) SEP_BDPP("%14", "%16", "%15", 14)
+        : "+v"(a0), "+v"(b0), "+v"(a1), "+v"(b1), "+v"(a2), "+v"(b2), "+v"(a3), "+v"(b3),
+          "+v"(a4), "+v"(b4), "+v"(a5), "+v"(b5), "+v"(a6), "+v"(b6), "+v"(a7), "+v"(b7)
+        : "v"(cf));
 }
-int tune_batched_f64(sep_handle*, int64_t, const int64_t*, const int64_t*, const int64_t*,
-                     const double*, const double*, int32_t*)
+
+template <int W, int s, int l0>
+__device__ __forceinline__ void shear8_at(double (&w)[W], double cf)
 {
-    set_error("sep_tune_batched_f64: not built yet");
-    return SEP_ERR_UNSUPPORTED;
+#define BSL(l, o) w[(((s) - 2 * ((l0) + (l)) + (o)) % W + W) % W]
+    shear8(BSL(0, 0), BSL(0, 1), BSL(1, 0), BSL(1, 1), BSL(2, 0), BSL(2, 1), BSL(3, 0),
+           BSL(3, 1), BSL(4, 0), BSL(4, 1), BSL(5, 0), BSL(5, 1), BSL(6, 0), BSL(6, 1),
+           BSL(7, 0), BSL(7, 1), cf);
+#undef BSL
+}
+
+template <int N, int i0>
+struct Unroll {
+    template <class F>
+    __device__ __forceinline__ static void run(F&& f)
+    {
+        if constexpr (i0 < N) {
+            f(std::integral_constant<int, i0>{});
+            Unroll<N, i0 + 1>::run(f);
+        }
+    }
+};
+
+// Scratch of one persistent wave.
+struct Slot {
+    double* tab;     // time-major table, tab_rows x MW pairs
+    int* lt;         // layer_thickness, nmax
+    int64_t* st;     // chase state words
+};
+
+// One pass of KF layers [L0, L0 + KF) of the block over Q (n x n, ld n), 64 rows at a time.
+// MW is the table row width in pairs (a multiple of KF).
+template <int KF>
+__device__ void apply_pass(double* __restrict__ Q, int n, const double* __restrict__ tab, int MW,
+                           int L0, int lane)
+{
+    constexpr int W = 2 * KF, NRW = KF / 8, P = kPF;
+    static_assert(W % P == 0, "prefetch ring must divide the window period");
+    const int cl = lane & 15;
+    const int T = n + 2 * KF - 2;
+    const int nblk = (T + W - 1) / W;
+    // coefficient row of time tau: table row tau + 2 L0, pairs L0 .. L0 + KF - 1
+    const double* __restrict__ cbase = tab + (int64_t)(2 * L0) * MW * 2 + 2 * L0 + cl;
+    const int64_t crow = (int64_t)MW * 2;
+    for (int r0 = 0; r0 < n; r0 += kLanes) {
+        const int row = min(r0 + lane, n - 1);
+        double* __restrict__ Qr = Q + row;
+        double win[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) win[w] = 0.0;
+        win[0] = Qr[0];
+        double cq[P][NRW], qi[P];
+#pragma unroll
+        for (int s = 0; s < P; ++s) {
+#pragma unroll
+            for (int r = 0; r < NRW; ++r) cq[s][r] = cbase[s * crow + 16 * r];
+            qi[s] = Qr[(int64_t)min(s + 1, n - 1) * n];
+        }
+        for (int b = 0; b < nblk; ++b) {
+            const int tau0 = b * W;
+            Unroll<W, 0>::run([&](auto sc) {
+                constexpr int s = decltype(sc)::value;
+                constexpr int k = s % P;
+                const int tau = tau0 + s;
+                win[(s + 1) % W] = qi[k];
+                shear8_at<W, s, 0>(win, cq[k][0]);
+                if constexpr (NRW > 1) shear8_at<W, s, 8>(win, cq[k][1]);
+                // refill this ring slot with time tau + P
+#pragma unroll
+                for (int r = 0; r < NRW; ++r) cq[k][r] = cbase[(int64_t)(tau + P) * crow + 16 * r];
+                qi[k] = Qr[(int64_t)min(tau + P + 1, n - 1) * n];
+                const int col = tau - 2 * KF + 2;
+                if (col >= 0 && col < n) Qr[(int64_t)col * n] = win[(s + 2) % W];
+            });
+        }
+    }
+}
+
+__device__ __forceinline__ int wave_bcast(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// qr_algorithm! on one problem by one wave (src/bulge_chasing.jl:146-288).  ML is the
+// sweep block length (max_layers), MW >= ML the table row width in pairs.
+template <int KF>
+__device__ void solve_one(double* dv, double* ev, double* Q, int n, int ML, int MW, int tab_rows,
+                          const Slot& sl, int32_t* conv_out, int64_t* iter_out, int lane)
+{
+    using chase::ST_DID;
+    using chase::ST_EARLY;
+    using chase::ST_ITER;
+    using chase::ST_LBEA;
+    using chase::ST_MAXLAYER;
+    using chase::ST_TO;
+    const double tol = 2.220446049250313e-16;  // eps(Float64)
+    const int64_t maxiter = 100 * (int64_t)n;
+    if (lane == 0) {
+        sl.st[ST_TO] = n;
+        sl.st[ST_LBEA] = n;
+        sl.st[ST_ITER] = 0;
+    }
+    __threadfence_block();
+    chase::Problem<double> P{dv, ev, sl.lt, sl.tab, n, ML, 0, MW};
+    int conv = 1;
+    const int64_t tab_pairs = (int64_t)tab_rows * MW;
+    for (;;) {
+        // unwritten table entries must read as (0, 0), the identity in shear form
+        for (int64_t q = lane; q < tab_pairs; q += kLanes)
+            *reinterpret_cast<double2*>(sl.tab + 2 * q) = make_double2(0.0, 0.0);
+        __threadfence_block();
+        chase::chase_block(P, tol, maxiter, sl.st, lane);
+        __threadfence_block();
+        const int early = wave_bcast((int)sl.st[ST_EARLY]);
+        const int did = wave_bcast((int)sl.st[ST_DID]);
+        if (early) {
+            conv = 0;
+            break;
+        }
+        if (!did) break;
+        const int depth = wave_bcast((int)sl.st[ST_MAXLAYER]);
+        // (c, s) -> (p, q) over the rows the block can have written
+        const int rows = n + 2 * depth;
+        for (int q = lane; q < rows * depth; q += kLanes) {
+            const int t = q / depth, L = q % depth;
+            double2* e = reinterpret_cast<double2*>(sl.tab + 2 * ((int64_t)t * MW + L));
+            const double2 cs = *e;
+            *e = make_double2(cs.y / (1.0 + cs.x), -cs.y);
+        }
+        __threadfence_block();
+        for (int L0 = 0; L0 < depth; L0 += KF) apply_pass<KF>(Q, n, sl.tab, MW, L0, lane);
+        __threadfence_block();
+    }
+    if (lane == 0) {
+        *conv_out = conv;
+        *iter_out = sl.st[ST_ITER];
+    }
+}
+
+template <int KF>
+__global__ void __launch_bounds__(64) k_batched(int B, const int* __restrict__ order,
+                                                const int* __restrict__ ns,
+                                                const int64_t* __restrict__ off_d,
+                                                const int64_t* __restrict__ off_q, double* dv,
+                                                double* ev, double* Q, int ML, int MW, int tab_rows,
+                                                double* tab, int* lt, int64_t* st, int nmax,
+                                                int* queue, int32_t* conv, int64_t* iters)
+{
+    const int lane = threadIdx.x;
+    const int slot = blockIdx.x;
+    Slot sl{tab + (int64_t)slot * tab_rows * MW * 2, lt + (int64_t)slot * nmax,
+            st + (int64_t)slot * chase::ST_WORDS};
+    for (;;) {
+        int p = 0;
+        if (lane == 0) p = atomicAdd(queue, 1);
+        p = __shfl(p, 0, kLanes);
+        p = wave_bcast(p);
+        if (p >= B) break;
+        const int prob = order[p];
+        const int n = ns[prob];
+        solve_one<KF>(dv + off_d[prob], ev + off_d[prob], Q + off_q[prob], n, ML, MW, tab_rows,
+                      sl, conv + prob, iters + prob, lane);
+    }
+}
+
+__global__ void k_reset_queue(int* q) { *q = 0; }
+
+// The persistent kernel below is switched off: a GPU run of it did not complete, and no
+// mistake that explains it has been found yet (the same device code, executed on the host
+// by a 64-lane lockstep emulation, finishes and matches the C restatement on that input).
+// Both entry points validate their arguments and then return SEP_ERR_UNSUPPORTED.
+constexpr bool kBatchedEnabled = false;
+constexpr const char* kBatchedDisabledWhy =
+    "the persistent batched solver is disabled in this build (a GPU run of it did not "
+    "complete and the cause is not identified); solve problems one at a time";
+
+struct Config {
+    int KF, ML, NW;
+};
+constexpr Config kConfigs[] = {
+    {16, 32, 4096}, {16, 64, 4096}, {16, 16, 4096}, {8, 32, 4096},  {8, 16, 4096},
+    {16, 32, 2048}, {16, 64, 2048}, {8, 32, 2048},  {16, 32, 8192}, {8, 16, 8192},
+};
+constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
+
+int tab_rows_for(int nmax, int ML, int KF) { return nmax + 2 * ML + 4 * KF + kPF + 8; }
+// table row width: max_layers rounded up to the fused depth (pairs beyond ML stay (0, 0))
+int row_width_for(int ML, int KF) { return (ML + KF - 1) / KF * KF; }
+
+struct Layout {
+    std::vector<int> order, ns32;
+    int nmax = 0;
+};
+
+int check_layout(int64_t B, const int64_t* n, const int64_t* off_d, const int64_t* off_q,
+                 Layout& L)
+{
+    if (B < 0 || (B > 0 && (!n || !off_d || !off_q))) {
+        set_error("batched: invalid batch description");
+        return SEP_ERR_ARG;
+    }
+    L.ns32.resize(B);
+    L.order.resize(B);
+    for (int64_t p = 0; p < B; ++p) {
+        if (n[p] < 2 || n[p] > 8192 || off_d[p] < 0 || off_q[p] < 0) {
+            set_error("batched: problem %lld has order %lld (2 <= n <= 8192) or bad offsets",
+                      (long long)p, (long long)n[p]);
+            return SEP_ERR_ARG;
+        }
+        if (n[p] % 2) {
+            set_error("batched: problem %lld has odd order %lld (the reference asserts iseven(n))",
+                      (long long)p, (long long)n[p]);
+            return SEP_ERR_ODD_N;
+        }
+        L.ns32[p] = (int)n[p];
+        L.nmax = std::max(L.nmax, (int)n[p]);
+    }
+    std::iota(L.order.begin(), L.order.end(), 0);
+    // longest-processing-time first: the apply costs ~n^3, the chase ~n^2
+    std::stable_sort(L.order.begin(), L.order.end(),
+                     [&](int a, int b) { return L.ns32[a] > L.ns32[b]; });
+    return SEP_OK;
+}
+
+// Upload the layout and scratch for `cfg`; returns device pointers through the handle.
+int prepare(sep_handle* h, int64_t B, const int64_t* off_d, const int64_t* off_q,
+            const Layout& L, const Config& cfg, int& tab_rows)
+{
+    tab_rows = tab_rows_for(L.nmax, cfg.ML, cfg.KF);
+    const size_t meta = sizeof(int) * 2 * B + sizeof(int64_t) * 2 * B + 64;
+    if (int rc = h->batch_meta.reserve(meta)) return rc;
+    char* m = static_cast<char*>(h->batch_meta.p);
+    int* d_order = reinterpret_cast<int*>(m);
+    int* d_ns = d_order + B;
+    int64_t* d_offd = reinterpret_cast<int64_t*>(m + ((sizeof(int) * 2 * B + 15) / 16) * 16);
+    int64_t* d_offq = d_offd + B;
+    int* d_queue = reinterpret_cast<int*>(d_offq + B);
+    (void)d_queue;
+    SEP_HIP(hipMemcpyAsync(d_order, L.order.data(), sizeof(int) * B, hipMemcpyHostToDevice,
+                           h->stream));
+    SEP_HIP(hipMemcpyAsync(d_ns, L.ns32.data(), sizeof(int) * B, hipMemcpyHostToDevice, h->stream));
+    SEP_HIP(hipMemcpyAsync(d_offd, off_d, sizeof(int64_t) * B, hipMemcpyHostToDevice, h->stream));
+    SEP_HIP(hipMemcpyAsync(d_offq, off_q, sizeof(int64_t) * B, hipMemcpyHostToDevice, h->stream));
+    const int NW = (int)std::min<int64_t>(cfg.NW, std::max<int64_t>(B, 1));
+    const size_t tab_bytes =
+        sizeof(double) * 2 * (size_t)tab_rows * row_width_for(cfg.ML, cfg.KF) * NW;
+    const size_t lt_bytes = sizeof(int) * (size_t)L.nmax * NW;
+    const size_t st_bytes = sizeof(int64_t) * chase::ST_WORDS * NW;
+    if (int rc = h->batch_scratch.reserve(tab_bytes + lt_bytes + st_bytes + 256)) return rc;
+    return SEP_OK;
+}
+
+int launch(sep_handle* h, hipStream_t s, int64_t B, double* dv, double* ev, double* Q,
+           const Layout& L, const Config& cfg, int tab_rows, int32_t* conv, int64_t* iters)
+{
+    char* m = static_cast<char*>(h->batch_meta.p);
+    int* d_order = reinterpret_cast<int*>(m);
+    int* d_ns = d_order + B;
+    int64_t* d_offd = reinterpret_cast<int64_t*>(m + ((sizeof(int) * 2 * B + 15) / 16) * 16);
+    int64_t* d_offq = d_offd + B;
+    int* d_queue = reinterpret_cast<int*>(d_offq + B);
+    const int NW = (int)std::min<int64_t>(cfg.NW, std::max<int64_t>(B, 1));
+    const int MW = row_width_for(cfg.ML, cfg.KF);
+    char* sc = static_cast<char*>(h->batch_scratch.p);
+    double* tab = reinterpret_cast<double*>(sc);
+    int* lt = reinterpret_cast<int*>(sc + sizeof(double) * 2 * (size_t)tab_rows * MW * NW);
+    int64_t* st = reinterpret_cast<int64_t*>(
+        reinterpret_cast<char*>(lt) + ((sizeof(int) * (size_t)L.nmax * NW + 15) / 16) * 16);
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, s, d_queue);
+    SEP_CHECK_LAUNCH();
+    if (cfg.KF == 8)
+        hipLaunchKernelGGL((k_batched<8>), dim3(NW), dim3(64), 0, s, (int)B, d_order, d_ns,
+                           d_offd, d_offq, dv, ev, Q, cfg.ML, MW, tab_rows, tab, lt, st, L.nmax,
+                           d_queue, conv, iters);
+    else
+        hipLaunchKernelGGL((k_batched<16>), dim3(NW), dim3(64), 0, s, (int)B, d_order, d_ns,
+                           d_offd, d_offq, dv, ev, Q, cfg.ML, MW, tab_rows, tab, lt, st, L.nmax,
+                           d_queue, conv, iters);
+    SEP_CHECK_LAUNCH();
+    return SEP_OK;
+}
+
+std::vector<int64_t> layout_key(int64_t B, const int64_t* n, const int64_t* off_d,
+                                const int64_t* off_q)
+{
+    std::vector<int64_t> k;
+    k.reserve(3 * B + 1);
+    k.push_back(B);
+    k.insert(k.end(), n, n + B);
+    k.insert(k.end(), off_d, off_d + B);
+    k.insert(k.end(), off_q, off_q + B);
+    return k;
+}
+
+__global__ void k_identity_batch(double* Q, const int* ns, const int64_t* off_q, int B)
+{
+    const int p = blockIdx.y;
+    if (p >= B) return;
+    const int n = ns[p];
+    double* q = Q + off_q[p];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < (int64_t)n * n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        q[e] = (e % (n + 1) == 0) ? 1.0 : 0.0;
+}
+
+}  // namespace
+
+int tune_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d,
+                     const int64_t* off_q, const double* dv, const double* ev, int32_t* choice)
+{
+    Layout L;
+    if (int rc = check_layout(B, n, off_d, off_q, L)) return rc;
+    if (B == 0) {
+        if (choice) *choice = 0;
+        return SEP_OK;
+    }
+    if (!kBatchedEnabled) {
+        set_error("batched: %s", kBatchedDisabledWhy);
+        return SEP_ERR_UNSUPPORTED;
+    }
+    // scratch copies of the inputs (the caller's arrays are not modified)
+    int64_t nd = 0, nq = 0;
+    for (int64_t p = 0; p < B; ++p) {
+        nd = std::max(nd, off_d[p] + n[p]);
+        nq = std::max(nq, off_q[p] + n[p] * n[p]);
+    }
+    DevBuf cd, ce, cq, cc, ci, cns, cofq;
+    int rc = SEP_OK;
+    auto fail = [&](int r) {
+        cd.release(); ce.release(); cq.release(); cc.release(); ci.release();
+        cns.release(); cofq.release();
+        return r;
+    };
+    if ((rc = cd.reserve(sizeof(double) * nd)) || (rc = ce.reserve(sizeof(double) * nd)) ||
+        (rc = cq.reserve(sizeof(double) * nq)) || (rc = cc.reserve(sizeof(int32_t) * B)) ||
+        (rc = ci.reserve(sizeof(int64_t) * B)) || (rc = cns.reserve(sizeof(int) * B)) ||
+        (rc = cofq.reserve(sizeof(int64_t) * B)))
+        return fail(rc);
+    SEP_HIP(hipMemcpyAsync(cns.p, L.ns32.data(), sizeof(int) * B, hipMemcpyHostToDevice, h->stream));
+    SEP_HIP(hipMemcpyAsync(cofq.p, off_q, sizeof(int64_t) * B, hipMemcpyHostToDevice, h->stream));
+    hipEvent_t e0, e1;
+    SEP_HIP(hipEventCreate(&e0));
+    SEP_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int best_i = 0;
+    for (int c = 0; c < kNumConfigs && rc == SEP_OK; ++c) {
+        int tab_rows = 0;
+        if ((rc = prepare(h, B, off_d, off_q, L, kConfigs[c], tab_rows))) break;
+        SEP_HIP(hipMemcpyAsync(cd.p, dv, sizeof(double) * nd, hipMemcpyDeviceToDevice, h->stream));
+        SEP_HIP(hipMemcpyAsync(ce.p, ev, sizeof(double) * nd, hipMemcpyDeviceToDevice, h->stream));
+        hipLaunchKernelGGL(k_identity_batch, dim3(64, (unsigned)B), dim3(256), 0, h->stream,
+                           cq.as<double>(), cns.as<int>(), cofq.as<int64_t>(), (int)B);
+        SEP_CHECK_LAUNCH();
+        SEP_HIP(hipEventRecord(e0, h->stream));
+        if ((rc = launch(h, h->stream, B, cd.as<double>(), ce.as<double>(), cq.as<double>(), L,
+                         kConfigs[c], tab_rows, cc.as<int32_t>(), ci.as<int64_t>())))
+            break;
+        SEP_HIP(hipEventRecord(e1, h->stream));
+        SEP_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        SEP_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            best_i = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    fail(SEP_OK);
+    // the tuning runs may have re-laid the scratch the cached graph points into
+    if (h->batched_exec) {
+        (void)hipGraphExecDestroy(h->batched_exec);
+        h->batched_exec = nullptr;
+        h->batched_key.clear();
+    }
+    if (rc) return rc;
+    h->tune_key = layout_key(B, n, off_d, off_q);
+    h->tune_choice = best_i;
+    if (choice) *choice = best_i;
+    return SEP_OK;
+}
+
+int qr_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d,
+                   const int64_t* off_q, double* dv, double* ev, double* Q, int64_t max_layers,
+                   int32_t* converged, int64_t* iters)
+{
+    Layout L;
+    if (int rc = check_layout(B, n, off_d, off_q, L)) return rc;
+    if (B == 0) return SEP_OK;
+    if (!dv || !ev || !Q || !converged || !iters) {
+        set_error("batched: null device pointer");
+        return SEP_ERR_ARG;
+    }
+    if (max_layers > 256) {
+        set_error("batched: max_layers=%lld above 256", (long long)max_layers);
+        return SEP_ERR_BAD_LAYERS;
+    }
+    if (!kBatchedEnabled) {
+        set_error("batched: %s", kBatchedDisabledWhy);
+        return SEP_ERR_UNSUPPORTED;
+    }
+    std::vector<int64_t> key = layout_key(B, n, off_d, off_q);
+    Config cfg = kConfigs[0];
+    if (h->tune_choice >= 0 && h->tune_key == key) cfg = kConfigs[h->tune_choice];
+    if (max_layers > 0) cfg.ML = (int)max_layers;
+    // graph cache: the queue reset and the persistent kernel, per layout / pointers / config
+    key.push_back(reinterpret_cast<int64_t>(dv));
+    key.push_back(reinterpret_cast<int64_t>(ev));
+    key.push_back(reinterpret_cast<int64_t>(Q));
+    key.push_back(reinterpret_cast<int64_t>(converged));
+    key.push_back(reinterpret_cast<int64_t>(iters));
+    key.push_back(cfg.KF);
+    key.push_back(cfg.ML);
+    key.push_back(cfg.NW);
+    if (!(h->batched_exec && h->batched_key == key)) {
+        if (h->batched_exec) {
+            (void)hipGraphExecDestroy(h->batched_exec);
+            h->batched_exec = nullptr;
+        }
+        int tab_rows = 0;
+        if (int rc = prepare(h, B, off_d, off_q, L, cfg, tab_rows)) return rc;
+        SEP_HIP(hipStreamSynchronize(h->stream));
+        hipGraph_t g = nullptr;
+        SEP_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        int rc = launch(h, h->stream, B, dv, ev, Q, L, cfg, tab_rows, converged, iters);
+        hipError_t ce = hipStreamEndCapture(h->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ce != hipSuccess) {
+            set_error("batched: graph capture failed: %s", hipGetErrorString(ce));
+            return SEP_ERR_HIP;
+        }
+        hipError_t ie = hipGraphInstantiate(&h->batched_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) {
+            h->batched_exec = nullptr;
+            set_error("batched: graph instantiate failed: %s", hipGetErrorString(ie));
+            return SEP_ERR_HIP;
+        }
+        h->batched_key = key;
+    }
+    SEP_HIP(hipGraphLaunch(h->batched_exec, h->stream));
+    return SEP_OK;
 }
 
 }  // namespace sep

@@ -242,8 +242,11 @@ int sep_create(sep_handle** out, int device)
         h->force_sgpr_apply = std::strcmp(ap, "sgpr") == 0;
         h->apply_geom = std::strcmp(ap, "deep") == 0 ? 1 : 0;
     }
-    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->chase_stream, hipStreamNonBlocking);
+    // Blocking streams: work on them is ordered after earlier work on the legacy default
+    // stream and before later work there, so device arrays a caller produced or consumes
+    // on the default stream need no extra synchronisation.
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamDefault);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->chase_stream, hipStreamDefault);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_a, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_b, hipEventDisableTiming);
     if (e != hipSuccess) {
@@ -339,7 +342,7 @@ int sep_set_stream(sep_handle* h, void* stream)
         h->stream = static_cast<hipStream_t>(stream);
         h->own_stream = false;
     } else if (!h->own_stream) {
-        SEP_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        SEP_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamDefault));
         h->own_stream = true;
     }
     return SEP_OK;

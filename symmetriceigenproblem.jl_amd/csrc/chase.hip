@@ -27,7 +27,7 @@ __global__ void __launch_bounds__(64) k_chase_block(T* dv, T* ev, int* lt, T* gr
                                                     int max_layers, T tol, int64_t maxiter,
                                                     int64_t* st)
 {
-    chase::Problem<T> P{dv, ev, lt, grid, n, max_layers, (int64_t)(n - 1)};
+    chase::Problem<T> P{dv, ev, lt, grid, n, max_layers, (int64_t)(n - 1), 0};
     chase::chase_block(P, tol, maxiter, st, (int)threadIdx.x);
 }
 

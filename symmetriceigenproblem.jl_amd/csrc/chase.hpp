@@ -223,8 +223,11 @@ __device__ __forceinline__ T wilkinson(T H11, T H12, T H22)
     return lam;
 }
 
-// Problem view.  dv (n), ev (n-1), lt (n), grid (n-1) x max_layers interleaved pairs
-// with leading dimension ldg pairs.  Indices in the member functions are 1-based.
+// Problem view.  dv (n), ev (n-1), lt (n) and the (c,s) grid.  With tm == 0 the grid is
+// the reference's (n-1) x max_layers matrix of interleaved pairs (leading dimension ldg
+// pairs); with tm == max_layers > 0 it is time-major for the in-wave apply: rotation
+// (i, L) (0-based) at row i + 2L, column L of rows of tm pairs.  Indices in the member
+// functions are 1-based.
 template <typename T>
 struct Problem {
     T* dv;
@@ -234,7 +237,16 @@ struct Problem {
     int n;
     int max_layers;
     int64_t ldg;
+    int tm;
 };
+
+// Pair index of rotation (i, layer) (both 1-based) in the grid.
+template <typename T>
+__device__ __forceinline__ int64_t grid_pair(const Problem<T>& P, int i, int layer)
+{
+    if (P.tm) return ((int64_t)(i - 1) + 2 * (int64_t)(layer - 1)) * P.tm + (layer - 1);
+    return (int64_t)(i - 1) + P.ldg * (layer - 1);
+}
 
 // The highest 1-based q < to with is_small(q), scanning down to `start`, or start-1.
 template <typename T>
@@ -275,7 +287,6 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
     T* __restrict__ ev = P.ev;
     int* __restrict__ lt = P.lt;
     T* __restrict__ grid = P.grid;
-    const int64_t ldg = P.ldg;
     // ---- first rotation, :17-63.  x*y + z*w is evaluated as fma(x, y, z*w) (the
     // reference's single_shift! is @fastmath; the oracle contracts identically).
     const T H11 = dv[from - 1];
@@ -302,7 +313,7 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
         const int layer = __builtin_amdgcn_readfirstlane(lt[from - 1]) + 1;
         dv[from - 1] = H11pp;
         lt[from - 1] = layer;
-        store_pair(grid + 2 * ((int64_t)(from - 1) + ldg * (layer - 1)), c1, s1);
+        store_pair(grid + 2 * grid_pair(P, from, layer), c1, s1);
     }
     // carried state entering step i: p1 = ev[i-1], d_cur = dv[i], e_cur = ev[i]
     T p1 = H12pp;
@@ -356,7 +367,7 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                 ev[i - 2] = G.r;   // ev[i-1] (1-based) = nrm
                 dv[i - 1] = B11pp; // dv[i]   (1-based) = B11''
                 lt[i - 1] = layer;
-                store_pair(grid + 2 * ((int64_t)(i - 1) + ldg * (layer - 1)), c, s);
+                store_pair(grid + 2 * grid_pair(P, i, layer), c, s);
                 p1 = B12pp;
                 d_cur = B22pp;
                 if (i + 2 <= to) {
