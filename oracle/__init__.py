@@ -187,3 +187,76 @@ def chase_block(dv, ev, state, max_layers: int, tol, maxiter: int):
     _fn("orc_chase_block", dv.dtype)(n, _ptr(dv), _ptr(ev), max_layers, tol, maxiter,
                                      _ptr(grid), _ptr(lt), _ptr(state))
     return grid, lt
+
+
+# ---------------------------------------------------------------- backward (added capability)
+# There is no reference counterpart: the eigendecomposition derivative is the textbook
+# adjoint of T = Q diag(lam) Q^T restricted to the tridiagonal parameters (d, e).  It is
+# pinned in tests/test_oracle.py by central finite differences of a LAPACK solve.
+
+def eig_backward_dense(lam, Q, lam_bar, Q_bar):
+    """(d_bar, e_bar) for cotangents of (lam, Q), by the dense formula
+    T_bar = Q (diag(lam_bar) + F o (Q^T Q_bar)) Q^T, F_ij = 1/(lam_j - lam_i), F_ii = 0;
+    d_bar = diag(T_bar), e_bar_k = T_bar[k, k+1] + T_bar[k+1, k]."""
+    lam = np.asarray(lam, dtype=np.float64)
+    Q = np.asarray(Q, dtype=np.float64)
+    n = lam.shape[0]
+    W = Q.T @ np.asarray(Q_bar, dtype=np.float64)
+    D = lam[None, :] - lam[:, None]
+    np.fill_diagonal(D, 1.0)
+    F = 1.0 / D
+    np.fill_diagonal(F, 0.0)
+    M = F * W
+    M[np.diag_indices(n)] = np.asarray(lam_bar, dtype=np.float64)
+    Tb = Q @ M @ Q.T
+    d_bar = np.diag(Tb).copy()
+    e_bar = np.diag(Tb, 1) + np.diag(Tb, -1)
+    return d_bar, e_bar
+
+
+def eig_sorted_signfixed(d, e):
+    """LAPACK eigendecomposition of T(d, e), eigenvalues ascending, each eigenvector's
+    largest-magnitude entry made positive (a locally constant convention, so the loss
+    L = w.lam + <G, Q> is differentiable under small perturbations)."""
+    import scipy.linalg as sl
+    lam, Q = sl.eigh_tridiagonal(np.asarray(d, dtype=np.float64), np.asarray(e, dtype=np.float64))
+    idx = np.argmax(np.abs(Q), axis=0)
+    sgn = np.sign(Q[idx, np.arange(Q.shape[1])])
+    sgn[sgn == 0] = 1.0
+    return lam, np.asfortranarray(Q * sgn[None, :])
+
+
+def signfix_like(Q, Qref):
+    """Flip columns of Q to agree in sign with Qref (same eigenvectors up to sign)."""
+    s = np.sign(np.sum(Q * Qref, axis=0))
+    s[s == 0] = 1.0
+    return np.asfortranarray(Q * s[None, :])
+
+
+def fd_gradient(d, e, w, G, h: float = 1e-6, d_idx=None, e_idx=None):
+    """Central differences of L(d, e) = w.lam + <G, Q> (sorted, sign-fixed LAPACK solve)
+    on the chosen entries of d and e (all by default)."""
+    d = np.asarray(d, dtype=np.float64)
+    e = np.asarray(e, dtype=np.float64)
+    _, Q0 = eig_sorted_signfixed(d, e)
+
+    def loss(dd, ee):
+        lam, Q = eig_sorted_signfixed(dd, ee)
+        Q = signfix_like(Q, Q0)
+        return float(w @ lam + np.sum(G * Q))
+
+    d_idx = range(d.shape[0]) if d_idx is None else d_idx
+    e_idx = range(e.shape[0]) if e_idx is None else e_idx
+    gd = {}
+    for k in d_idx:
+        dp, dm = d.copy(), d.copy()
+        dp[k] += h
+        dm[k] -= h
+        gd[k] = (loss(dp, e) - loss(dm, e)) / (2 * h)
+    ge = {}
+    for k in e_idx:
+        ep, em = e.copy(), e.copy()
+        ep[k] += h
+        em[k] -= h
+        ge[k] = (loss(d, ep) - loss(d, em)) / (2 * h)
+    return gd, ge

@@ -1,20 +1,315 @@
-// backward.hip -- placeholder until the MFMA backward lands.
+// backward.hip -- reverse-mode derivative of the eigendecomposition of a symmetric
+// tridiagonal T(d, e) (no counterpart in the reference, which has no AD support).
+//
+// With T = Q diag(lam) Q^T and cotangents (lam_bar, Q_bar):
+//   W = Q^T Q_bar,  M = diag(lam_bar) + F o W,  F_ij = 1/(lam_j - lam_i) (i != j), 0 (i == j)
+//   T_bar = Q M Q^T,  d_bar_k = T_bar_kk,  e_bar_k = T_bar_{k,k+1} + T_bar_{k+1,k}.
+// T_bar is never formed.  With P = Q M, d_bar_k = <P_k., Q_k.>, and e_bar_k =
+// <P_k., Q_{k+1}.> + <P_{k+1}., Q_k.>, so every row k of P contributes three dot products
+// with rows k-1, k, k+1 of Q; those are taken in the epilogue of the second product, and P
+// is never written either.
+//
+// Kernels (Float64 MFMA, v_mfma_f64_16x16x4f64):
+//   k_bwd_gemm<kModeM>: W tile = Q^T Q_bar tile; epilogue writes M = F o W + diag(lam_bar)
+//   k_bwd_gemm<kModeP>: P tile = Q M tile; epilogue reduces the tile's three row dot-product
+//                       partial sums and stores them per column tile (no atomics)
+//   k_bwd_reduce:       sums the column-tile partials in a fixed order -> d_bar, e_bar
+// Block tile 128 x 128, k step 16, 256 threads = 2 x 2 waves of 64 x 64 (4 x 4 MFMA tiles of
+// 16 x 16, four f64 accumulators per lane each), double-buffered LDS, one barrier per k step.
+// Problems are described by a table, so the single and the batched entry points share the
+// kernels (grid z = problem; tiles beyond a problem's order exit at once).
 #include "sep_internal.hpp"
 
 namespace sep {
+namespace {
 
-int eig_backward_f64(sep_handle*, int64_t, const double*, const double*, int64_t, const double*,
-                     const double*, double*, double*)
+using f64x4 = double __attribute__((ext_vector_type(4)));
+
+constexpr int kBT = 128;      // block tile (rows = columns)
+constexpr int kBK = 16;       // k step
+constexpr int kLdsLd = kBT + 4;
+constexpr int kThreads = 256;
+
+struct BwdProb {
+    int n;
+    int ld;         // leading dimension of Q and Q_bar
+    int64_t off_q;  // offset of Q / Q_bar (elements)
+    int64_t off_d;  // offset of lam / lam_bar / d_bar / e_bar, and partial-sum column
+    int64_t off_m;  // offset of this problem's M (n x n, ld n) in the scratch
+};
+
+enum { kModeM = 0, kModeP = 1 };
+
+template <int MODE>
+__global__ void __launch_bounds__(kThreads)
+    k_bwd_gemm(const BwdProb* __restrict__ probs, const double* __restrict__ lam,
+               const double* __restrict__ lam_bar, const double* __restrict__ Q,
+               const double* __restrict__ Qb, double* __restrict__ M,
+               double* __restrict__ part, int64_t part_ld)
 {
-    set_error("sep_eig_backward_f64_dev: not built yet");
-    return SEP_ERR_UNSUPPORTED;
+    __shared__ double As[2][kBK][kLdsLd];
+    __shared__ double Bs[2][kBK][kLdsLd];
+
+    const BwdProb pr = probs[blockIdx.z];
+    const int n = pr.n, ld = pr.ld;
+    const int i0 = blockIdx.x * kBT, j0 = blockIdx.y * kBT;
+    if (i0 >= n || j0 >= n) return;
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    const int wr = wid >> 1, wc = wid & 1;
+    const double* __restrict__ Qp = Q + pr.off_q;
+    // B operand: Q_bar (ld) for the first product, M (ld n) for the second
+    const double* __restrict__ Bp = (MODE == kModeM) ? Qb + pr.off_q : M + pr.off_m;
+    const int ldb = (MODE == kModeM) ? ld : n;
+
+    double ra[8], rb[8];
+    // A[i][k] = Q[k + i ld] (first product, Q^T) or Q[i + k ld] (second product)
+    auto load_tile = [&](int k0) {
+        if constexpr (MODE == kModeM) {
+            const int i = i0 + (t >> 1), kb = k0 + (t & 1) * 8;
+            const double* src = Qp + (int64_t)i * ld + kb;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) ra[u] = (i < n && kb + u < n) ? src[u] : 0.0;
+        } else {
+            const int k = k0 + (t >> 4), ib = i0 + (t & 15) * 8;
+            const double* src = Qp + (int64_t)k * ld + ib;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) ra[u] = (k < n && ib + u < n) ? src[u] : 0.0;
+        }
+        const int j = j0 + (t >> 1), kb = k0 + (t & 1) * 8;
+        const double* src = Bp + (int64_t)j * ldb + kb;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) rb[u] = (j < n && kb + u < n) ? src[u] : 0.0;
+    };
+    auto store_tile = [&](int buf) {
+        if constexpr (MODE == kModeM) {
+            const int i = t >> 1, kb = (t & 1) * 8;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) As[buf][kb + u][i] = ra[u];
+        } else {
+            const int k = t >> 4, ib = (t & 15) * 8;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) As[buf][k][ib + u] = ra[u];
+        }
+        const int j = t >> 1, kb = (t & 1) * 8;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) Bs[buf][kb + u][j] = rb[u];
+    };
+
+    f64x4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+    const int nk = (n + kBK - 1) / kBK;
+    load_tile(0);
+    store_tile(0);
+    __syncthreads();
+    const int fr = lane & 15, fk = lane >> 4;  // fragment row / k of this lane
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) load_tile((kt + 1) * kBK);
+#pragma unroll
+        for (int ks = 0; ks < kBK / 4; ++ks) {
+            const int kr = ks * 4 + fk;
+            double a[4], b[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) a[r] = As[cur][kr][wr * 64 + r * 16 + fr];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) b[c] = Bs[cur][kr][wc * 64 + c * 16 + fr];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
+        }
+        if (kt + 1 < nk) store_tile(cur ^ 1);
+        __syncthreads();
+    }
+
+    // accumulator (r, c), element v of this lane: row fk + 4 v, column fr of the 16 x 16 tile
+    // (the v_mfma_f64_16x16x4f64 result layout: lane j + 16 (i mod 4), element i / 4)
+    if constexpr (MODE == kModeM) {
+        const double* __restrict__ lp = lam + pr.off_d;
+        const double* __restrict__ lbp = lam_bar + pr.off_d;
+        double* __restrict__ Mp = M + pr.off_m;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int j = j0 + wc * 64 + c * 16 + fr;
+            if (j >= n) continue;
+            const double lj = lp[j];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = i0 + wr * 64 + r * 16 + fk + 4 * v;
+                    if (i >= n) continue;
+                    Mp[i + (int64_t)j * n] = (i == j) ? lbp[i] : acc[r][c][v] / (lj - lp[i]);
+                }
+            }
+        }
+    } else {
+        // the LDS tiles are free after the last barrier: reuse As for the wave partials
+        double* red = &As[0][0][0];  // [2 wc][128 rows][3]
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int il = wr * 64 + r * 16 + fk + 4 * v;
+                const int i = i0 + il;
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int j = j0 + wc * 64 + c * 16 + fr;
+                    if (i < n && j < n) {
+                        const double p = acc[r][c][v];
+                        const double* qc = Qp + (int64_t)j * ld;
+                        s0 = __builtin_fma(p, qc[i], s0);
+                        if (i + 1 < n) s1 = __builtin_fma(p, qc[i + 1], s1);
+                        if (i > 0) s2 = __builtin_fma(p, qc[i - 1], s2);
+                    }
+                }
+#pragma unroll
+                for (int off = 8; off > 0; off >>= 1) {
+                    s0 += __shfl_xor(s0, off, 64);
+                    s1 += __shfl_xor(s1, off, 64);
+                    s2 += __shfl_xor(s2, off, 64);
+                }
+                if (fr == 0) {
+                    double* o = red + ((size_t)wc * kBT + il) * 3;
+                    o[0] = s0;
+                    o[1] = s1;
+                    o[2] = s2;
+                }
+            }
+        }
+        __syncthreads();
+        if (t < kBT) {
+            const int i = i0 + t;
+            if (i < n) {
+                const double* a = red + (size_t)t * 3;
+                const double* b = red + ((size_t)kBT + t) * 3;
+                double* dst = part + (int64_t)blockIdx.y * 3 * part_ld + pr.off_d + i;
+                dst[0] = a[0] + b[0];
+                dst[part_ld] = a[1] + b[1];
+                dst[2 * part_ld] = a[2] + b[2];
+            }
+        }
+    }
 }
-int eig_backward_batched_f64(sep_handle*, int64_t, const int64_t*, const int64_t*,
-                             const int64_t*, const double*, const double*, const double*,
-                             const double*, double*, double*)
+
+// d_bar_k = sum_ct S0[ct][k];  e_bar_k = sum_ct (S1[ct][k] + S2[ct][k+1])
+__global__ void __launch_bounds__(256)
+    k_bwd_reduce(const BwdProb* __restrict__ probs, const double* __restrict__ part,
+                 int64_t part_ld, double* __restrict__ d_bar, double* __restrict__ e_bar)
 {
-    set_error("sep_eig_backward_batched_f64_dev: not built yet");
-    return SEP_ERR_UNSUPPORTED;
+    const BwdProb pr = probs[blockIdx.y];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= pr.n) return;
+    const int nct = (pr.n + kBT - 1) / kBT;
+    const int64_t o = pr.off_d + k;
+    double d = 0.0, e = 0.0;
+    for (int ct = 0; ct < nct; ++ct) {
+        const double* p = part + (int64_t)ct * 3 * part_ld;
+        d += p[o];
+        if (k + 1 < pr.n) e += p[part_ld + o] + p[2 * part_ld + o + 1];
+    }
+    d_bar[o] = d;
+    if (k + 1 < pr.n) e_bar[o] = e;
+}
+
+int run_backward(sep_handle* h, const std::vector<BwdProb>& probs, int nmax, int64_t tot_d,
+                 int64_t tot_m, const double* lam, const double* Q, const double* lam_bar,
+                 const double* Q_bar, double* d_bar, double* e_bar)
+{
+    const int64_t B = (int64_t)probs.size();
+    if (B == 0 || nmax == 0) return SEP_OK;
+    if (B > 65535) {
+        set_error("eig_backward: at most 65535 problems per call (got %lld)", (long long)B);
+        return SEP_ERR_ARG;
+    }
+    const int nct = (nmax + kBT - 1) / kBT;
+    const size_t meta_bytes = sizeof(BwdProb) * (size_t)B;
+    if (int rc = h->bwd_meta.reserve(meta_bytes)) return rc;
+    if (int rc = h->bwd_M.reserve(sizeof(double) * (size_t)tot_m)) return rc;
+    if (int rc = h->bwd_part.reserve(sizeof(double) * 3 * (size_t)nct * (size_t)(tot_d + 1)))
+        return rc;
+    // the table is uploaded from a handle-owned host copy, and only when it changes
+    const char* raw = reinterpret_cast<const char*>(probs.data());
+    if (h->bwd_meta_host.size() != meta_bytes ||
+        std::memcmp(h->bwd_meta_host.data(), raw, meta_bytes) != 0 || h->bwd_meta_dev != h->bwd_meta.p) {
+        h->bwd_meta_host.assign(raw, raw + meta_bytes);
+        SEP_HIP(hipMemcpyAsync(h->bwd_meta.p, h->bwd_meta_host.data(), meta_bytes,
+                               hipMemcpyHostToDevice, h->stream));
+        SEP_HIP(hipStreamSynchronize(h->stream));
+        h->bwd_meta_dev = h->bwd_meta.p;
+    }
+    const BwdProb* dp = h->bwd_meta.as<BwdProb>();
+    const int64_t part_ld = tot_d + 1;
+    double* Mb = h->bwd_M.as<double>();
+    double* part = h->bwd_part.as<double>();
+    const dim3 grid(nct, nct, (unsigned)B);
+    hipLaunchKernelGGL((k_bwd_gemm<kModeM>), grid, dim3(kThreads), 0, h->stream, dp, lam,
+                       lam_bar, Q, Q_bar, Mb, part, part_ld);
+    SEP_CHECK_LAUNCH();
+    hipLaunchKernelGGL((k_bwd_gemm<kModeP>), grid, dim3(kThreads), 0, h->stream, dp, lam,
+                       lam_bar, Q, Q_bar, Mb, part, part_ld);
+    SEP_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_bwd_reduce, dim3((nmax + 255) / 256, (unsigned)B), dim3(256), 0,
+                       h->stream, dp, part, part_ld, d_bar, e_bar);
+    SEP_CHECK_LAUNCH();
+    return SEP_OK;
+}
+
+}  // namespace
+
+int eig_backward_f64(sep_handle* h, int64_t n, const double* lam, const double* Q, int64_t ldq,
+                     const double* lam_bar, const double* Q_bar, double* d_bar, double* e_bar)
+{
+    if (n < 0 || n > (int64_t)1 << 20 || ldq < std::max<int64_t>(n, 1)) {
+        set_error("sep_eig_backward: invalid n=%lld ldq=%lld", (long long)n, (long long)ldq);
+        return SEP_ERR_ARG;
+    }
+    if (n == 0) return SEP_OK;
+    if (!lam || !Q || !lam_bar || !Q_bar || !d_bar || (n > 1 && !e_bar)) {
+        set_error("sep_eig_backward: null device pointer");
+        return SEP_ERR_ARG;
+    }
+    std::vector<BwdProb> p(1);
+    p[0] = BwdProb{(int)n, (int)ldq, 0, 0, 0};
+    // e_bar has n - 1 entries: the reduction writes e_bar[k] only for k + 1 < n
+    return run_backward(h, p, (int)n, n, n * n, lam, Q, lam_bar, Q_bar, d_bar, e_bar);
+}
+
+int eig_backward_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d,
+                             const int64_t* off_q, const double* lam, const double* Q,
+                             const double* lam_bar, const double* Q_bar, double* d_bar,
+                             double* e_bar)
+{
+    if (B < 0 || (B > 0 && (!n || !off_d || !off_q))) {
+        set_error("sep_eig_backward_batched: invalid batch description");
+        return SEP_ERR_ARG;
+    }
+    if (B == 0) return SEP_OK;
+    if (!lam || !Q || !lam_bar || !Q_bar || !d_bar || !e_bar) {
+        set_error("sep_eig_backward_batched: null device pointer");
+        return SEP_ERR_ARG;
+    }
+    std::vector<BwdProb> p((size_t)B);
+    int nmax = 0;
+    int64_t tot_d = 0, tot_m = 0;
+    for (int64_t i = 0; i < B; ++i) {
+        if (n[i] < 1 || n[i] > 65536 || off_d[i] < 0 || off_q[i] < 0) {
+            set_error("sep_eig_backward_batched: problem %lld has order %lld or bad offsets",
+                      (long long)i, (long long)n[i]);
+            return SEP_ERR_ARG;
+        }
+        p[i] = BwdProb{(int)n[i], (int)n[i], off_q[i], off_d[i], tot_m};
+        tot_m += n[i] * n[i];
+        tot_d = std::max(tot_d, off_d[i] + n[i]);
+        nmax = std::max(nmax, (int)n[i]);
+    }
+    return run_backward(h, p, nmax, tot_d, tot_m, lam, Q, lam_bar, Q_bar, d_bar, e_bar);
 }
 
 }  // namespace sep

@@ -279,7 +279,9 @@ int sep_destroy(sep_handle* h)
     h->lt.release();
     h->batch_meta.release();
     h->batch_scratch.release();
-    h->bwd_W.release();
+    h->bwd_meta.release();
+    h->bwd_M.release();
+    h->bwd_part.release();
     for (auto e : h->tev) (void)hipEventDestroy(e);
     if (h->ev_a) (void)hipEventDestroy(h->ev_a);
     if (h->ev_b) (void)hipEventDestroy(h->ev_b);

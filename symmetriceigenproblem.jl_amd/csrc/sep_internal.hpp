@@ -72,13 +72,6 @@ struct ApplyWork {
     DevBuf aux;  // column sign masks, output signs, normalisation flag
 };
 
-// Batched-solver tuning entry (chosen by the autotuner).
-struct BatchedTune {
-    int rows_per_lane;   // rows of Q each lane owns in the in-wave apply (1, 2, 4)
-    int fused_depth;     // rotation layers fused per pass over Q
-    int sweep_block;     // max_layers: sweeps accumulated per block
-};
-
 }  // namespace sep
 
 struct sep_handle {
@@ -92,11 +85,13 @@ struct sep_handle {
     sep::ApplyWork apply_work[2];
     sep::DevBuf grid[2], state, counters, lt;
     sep::DevBuf batch_meta, batch_scratch;
-    sep::DevBuf bwd_W;
+    // backward: problem table (device + the host copy it was uploaded from), M, partials
+    sep::DevBuf bwd_meta, bwd_M, bwd_part;
+    std::vector<char> bwd_meta_host;
+    void* bwd_meta_dev = nullptr;
     // batched graph cache
     hipGraphExec_t batched_exec = nullptr;
     std::vector<int64_t> batched_key;
-    int batched_choice = -1;
     std::vector<int64_t> tune_key;
     int tune_choice = -1;
     bool force_sgpr_apply = false;  // SEP_APPLY=sgpr selects the scalar-fed Float64 kernel

@@ -17,6 +17,12 @@ def dense(d, e):
     return np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
 
 
+def sep_inputs():
+    """The package's seeded input generators (pure numpy; no GPU needed)."""
+    import conftest
+    return conftest.load_package().inputs
+
+
 # ---------------------------------------------------------------- givensAlgorithm (DLARTG)
 
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 4e-16), (np.float32, 2e-7)])
@@ -280,3 +286,33 @@ def test_chase_block_resumes_to_full_solve():
         blocks += 1
         rots += st[5]
     assert np.array_equal(dv1, dv2) and st[2] == it and blocks == nblk and rots == nrot
+
+
+# ---------------------------------------------------------------- backward formula
+
+@pytest.mark.parametrize("n", [2, 3, 12, 40])
+def test_backward_dense_formula_matches_finite_differences(n):
+    """The dense adjoint formula against central differences (h = 1e-6) of a LAPACK
+    solve, on every entry of d and e, for L = w.lam + <G, Q> (d_i = i, e ~ U[0.1, 1))."""
+    d, e = sep_inputs().tridiag_linear(n, seed=17, e_low=0.1)
+    w, G = sep_inputs().backward_loss_weights(n, seed=17)
+    lam, Q = oracle.eig_sorted_signfixed(d, e)
+    db, eb = oracle.eig_backward_dense(lam, Q, w, G)
+    gd, ge = oracle.fd_gradient(d, e, w, G)
+    fd_d = np.array([gd[k] for k in range(n)])
+    fd_e = np.array([ge[k] for k in range(n - 1)])
+    scale = max(np.abs(fd_d).max(), np.abs(fd_e).max() if n > 1 else 0.0, 1.0)
+    assert np.abs(db - fd_d).max() <= 1e-6 * scale
+    assert np.abs(eb - fd_e).max() <= 1e-6 * scale
+
+
+def test_backward_dense_formula_eigenvalue_only_closed_form():
+    """With Q_bar = 0 the adjoint of lam_k is q_k q_k^T: d_bar = sum_k w_k q_k^2 elementwise
+    and e_bar_j = 2 sum_k w_k q_jk q_(j+1)k (Hellmann-Feynman)."""
+    n = 9
+    d, e = sep_inputs().tridiag_linear(n, seed=5, e_low=0.1)
+    lam, Q = oracle.eig_sorted_signfixed(d, e)
+    w = np.linspace(-1.0, 2.0, n)
+    db, eb = oracle.eig_backward_dense(lam, Q, w, np.zeros((n, n)))
+    assert np.allclose(db, (Q ** 2) @ w, rtol=0, atol=1e-14)
+    assert np.allclose(eb, 2 * (Q[:-1] * Q[1:]) @ w, rtol=0, atol=1e-14)
