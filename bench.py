@@ -12,10 +12,13 @@ Also reported in the same line:
   records on the stream the kernel runs on;
 * cpu_baseline: the oracle's wave-order apply (a C restatement of src/wave.jl and
   src/bulk_givens.jl) timed on this host's cores on a row sample of the same workload;
-* extra: the plain solve of the README example (n = 2000), and -- when built -- batched
-  solves/s and the backward/forward time ratio.
+* extra: the other apply shapes (n = 2000 Float64/Float32, n = 16384 Float32), the plain
+  solve of the README example (n = 2000) beside the oracle's CPU solve, the
+  differentiated-solve workload (n = 4096: forward solve, backward, and their ratio), and
+  the batched solver's status.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
+                      [--calibrate-copy-gib G]
 """
 from __future__ import annotations
 
@@ -30,7 +33,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "symmetriceigenproblem.jl_amd")
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix) peak
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak
 HBM_PEAK_GBS = 8000.0
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "apply_dpp_n16384_k64_f64_pmc.md")
+
+
+def pmc_traffic_bytes():
+    """HBM bytes per launch of the apply kernel, from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a plain-copy calibration); None when absent."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1].split()[0])
+    except OSError:
+        pass
+    return None
 
 
 def load_package():
@@ -64,17 +82,104 @@ def cpu_baseline(n: int, k: int, rows: int, threads: int, reps: int):
     return flops / best / 1e9, best
 
 
+def time_apply(sep, h, torch, n, k, dtype, steps=5):
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    A = torch.randn((n, n), generator=gen, dtype=dtype, device="cuda").t()
+    grid = torch.from_numpy(sep.inputs.givens_grid(n, k, seed=6)).to(dtype).cuda()
+    sep.bulk_wave_order_2x2_rmul_(A, grid, handle=h)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        sep.bulk_wave_order_2x2_rmul_(A, grid, handle=h)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3 / steps
+    return 6.0 * n * (n - 1) * k / (ms * 1e-3) / 1e9, ms
+
+
+def extras(sep, h, torch, np, cpu=True):
+    """Secondary figures: other apply shapes, config-1 solve, the differentiated solve."""
+    sys.path.insert(0, ROOT)
+    ex = {}
+    for n, dt, name in ((2000, torch.float64, "apply_n2000_f64"),
+                        (2000, torch.float32, "apply_n2000_f32"),
+                        (16384, torch.float32, "apply_n16384_f32")):
+        gfs, ms = time_apply(sep, h, torch, n, 64, dt)
+        peak = FP64_PEAK_TFLOPS if dt == torch.float64 else FP32_PEAK_TFLOPS
+        ex[name + "_gflops"] = round(gfs, 1)
+        ex[name + "_frac_of_vector_peak"] = round(gfs / 1e3 / peak, 4)
+        log(f"[bench] {name}: {gfs:.1f} GFLOP/s ({ms:.3f} ms/step)")
+
+    # config 1: README example solve, n = 2000, max_layers = 128
+    d, e = sep.inputs.tridiag_linear(2000)
+    H = sep.SymTridiagonal(torch.from_numpy(d).cuda(), torch.from_numpy(e).cuda())
+    Q = torch.eye(2000, dtype=torch.float64, device="cuda").t()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    _, conv, it, nrot = sep.qr_algorithm_(H, Q, handle=h, return_rotations=True)
+    torch.cuda.synchronize()
+    ex["solve_n2000_s"] = round(time.perf_counter() - t, 4)
+    ex["solve_n2000_iter"] = it
+    ex["solve_n2000_rotations"] = nrot
+    ex["solve_n2000_apply_gflops"] = round(6.0 * 2000 * nrot / ex["solve_n2000_s"] / 1e9, 1)
+    log(f"[bench] solve n=2000: {ex['solve_n2000_s']} s, iter={it}, rotations={nrot}")
+    if cpu:
+        import oracle
+        dv, ev = d.copy(), e.copy()
+        Qh = np.asfortranarray(np.eye(2000))
+        t = time.perf_counter()
+        oracle.qr_algorithm(dv, ev, Qh, 128, fast=True)
+        ex["solve_n2000_cpu_port_s"] = round(time.perf_counter() - t, 3)
+        ex["solve_n2000_cpu_port_note"] = ("oracle qr_algorithm (C restatement, FMA build), "
+                                           "1 thread, chase + layer-major apply")
+        log(f"[bench] solve n=2000 on the CPU port: {ex['solve_n2000_cpu_port_s']} s")
+
+    # config 5: differentiated solve, n = 4096, d_i = i, e ~ U[0.1, 1)
+    n = 4096
+    d, e = sep.inputs.tridiag_linear(n, seed=1, e_low=0.1)
+    w, G = sep.inputs.backward_loss_weights(n)
+    H = sep.SymTridiagonal(torch.from_numpy(d).cuda(), torch.from_numpy(e).cuda())
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    lam, Q = sep.eigen_decomp_(H, handle=h)
+    torch.cuda.synchronize()
+    fwd = time.perf_counter() - t
+    wd = torch.from_numpy(w).cuda()
+    Gd = sep.to_device_colmajor(G)
+    sep.eig_backward(lam, Q, wd, Gd, handle=h)
+    torch.cuda.synchronize()
+    reps = 5
+    t = time.perf_counter()
+    for _ in range(reps):
+        sep.eig_backward(lam, Q, wd, Gd, handle=h)
+    torch.cuda.synchronize()
+    bwd = (time.perf_counter() - t) / reps
+    ex["ad_n4096_forward_s"] = round(fwd, 4)
+    ex["ad_n4096_backward_s"] = round(bwd, 5)
+    ex["ad_n4096_bwd_over_fwd"] = round(bwd / fwd, 5)
+    ex["ad_n4096_backward_tflops"] = round(4.0 * n ** 3 / bwd / 1e12, 2)
+    log(f"[bench] AD n=4096: forward {fwd:.3f} s, backward {bwd * 1e3:.2f} ms "
+        f"({ex['ad_n4096_backward_tflops']} TFLOP/s on 4 n^3), ratio {bwd / fwd:.5f}")
+
+    ex["batched"] = {"status": "disabled", "reason": "the persistent batched kernel is "
+                     "switched off in this build (a GPU run did not complete; cause not found)"}
+    return ex
+
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=2048)
-    ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--cpu-rows", type=int, default=16384)
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--calibrate-copy-gib", type=float, default=0.0,
+                    help="after the timed steps, run one plain device copy of this many GiB "
+                         "(the reference kernel for FETCH_SIZE / WRITE_SIZE calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -144,7 +249,9 @@ def main() -> int:
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-            "traffic": None,
+            "traffic": pmc_traffic_bytes(),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE x 1)",
+            "algorithmic_bytes_per_launch": 2.0 * 8 * m * n + 16.0 * (n - 1) * k,
             "kernel": "k_apply_dpp<8,16>",
             "kernel_ms": round(kern_ms, 4),
             "algorithmic_flops_per_launch": flops * args.steps / max(launches, 1),
@@ -163,21 +270,18 @@ def main() -> int:
         }
         log(f"[bench] cpu {gfs:.1f} GFLOP/s")
 
+    if args.calibrate_copy_gib > 0:
+        nel = int(args.calibrate_copy_gib * (1 << 30)) // 8
+        src = torch.ones(nel, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy of {args.calibrate_copy_gib} GiB done")
+        del src, dst
+
     if not args.no_extra:
-        extra = {}
-        # plain solve, README example (n = 2000)
-        d, e = sep.inputs.tridiag_linear(2000)
-        H = sep.SymTridiagonal(torch.from_numpy(d).cuda(), torch.from_numpy(e).cuda())
-        Q = torch.eye(2000, dtype=torch.float64, device="cuda").t()
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        _, conv, it, nrot = sep.qr_algorithm_(H, Q, handle=h, return_rotations=True)
-        torch.cuda.synchronize()
-        extra["solve_n2000_s"] = round(time.perf_counter() - t, 4)
-        extra["solve_n2000_iter"] = it
-        extra["solve_n2000_rotations"] = nrot
-        log(f"[bench] solve n=2000: {extra['solve_n2000_s']} s, iter={it}")
-        out["extra"] = extra
+        out["extra"] = extras(sep, h, torch, np, cpu=not args.no_cpu)
 
     print(json.dumps(out), flush=True)
     h.close()

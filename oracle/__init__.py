@@ -148,11 +148,13 @@ def bulk_wave(A: np.ndarray, grid: np.ndarray, threads: int = 1, fast: bool = Fa
         raise AssertionError("wave schedule precondition violated (n-1 odd, k even, k <= n-2)")
 
 
-def qr_algorithm(dv, ev, Q=None, max_layers: int = 128, tol=None, maxiter=None, apply_mode: int = 0):
+def qr_algorithm(dv, ev, Q=None, max_layers: int = 128, tol=None, maxiter=None, apply_mode: int = 0,
+                 fast: bool = False):
     """qr_algorithm!(H, Q, max_layers, tol, maxiter); mutates dv, ev, Q.
 
     Returns (converged, iter, n_rotations, n_blocks).  apply_mode 0 = layer-major grid,
-    1 = wave order, 2 = chronological rotation log."""
+    1 = wave order, 2 = chronological rotation log.  fast=True runs the FMA-contracted build
+    (the timed CPU baseline; parity checks use the default build)."""
     dt = dv.dtype
     n = dv.shape[0]
     if tol is None:
@@ -168,7 +170,7 @@ def qr_algorithm(dv, ev, Q=None, max_layers: int = 128, tol=None, maxiter=None, 
     if Q is not None:
         assert Q.flags.f_contiguous and Q.dtype == dt and Q.shape == (n, n)
         qp = _ptr(Q)
-    rc = _fn("orc_qr_algorithm", dt)(n, _ptr(dv), _ptr(ev), qp, ldq, max_layers, tol, maxiter,
+    rc = _fn("orc_qr_algorithm", dt, fast)(n, _ptr(dv), _ptr(ev), qp, ldq, max_layers, tol, maxiter,
                                      C.byref(conv), C.byref(it), C.byref(nrot), apply_mode,
                                      C.byref(nblk))
     if rc == -1:

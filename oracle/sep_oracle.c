@@ -17,8 +17,13 @@
  *
  * Pinning (what checks each function beyond retyping the formula) lives in
  * tests/test_oracle.py: DLARTG identities and sign rule, LAPACK eigenvalues,
- * Toeplitz closed forms, the 2x2 closed form, bitwise layer-major == chronological
- * == wave order, residual and orthogonality bounds.
+ * Toeplitz closed forms, the 2x2 closed form, a dense shifted QR step, bitwise
+ * layer-major == wave order, residual and orthogonality bounds.
+ *
+ * Parity unpinned against the Julia reference itself (it cannot run here and holds no
+ * golden vectors): the exact iteration count, the deflation order of the eigenvalues,
+ * the bitwise grid contents and the rounding under @fastmath contraction are what this
+ * restatement reads in the reference's source, not values checked against it.
  *
  * Build: oracle/Makefile (two variants of the same source: -ffp-contract=off for
  * parity, and FMA contraction for the timed CPU baseline, like the reference's
