@@ -36,8 +36,8 @@ extern "C" {
 
 typedef struct sep_handle sep_handle;
 
-/* Library handle: owns the device buffers, two HIP streams (chase / apply),
- * the autotune table and the captured batched graphs.  One handle per host thread. */
+/* Library handle: owns the device buffers, two HIP streams (chase / apply) and the
+ * batched solver's autotuned choice.  One handle per host thread. */
 int sep_create(sep_handle **h, int device);
 int sep_destroy(sep_handle *h);
 /* Message of the last failing call on this thread ("" if none). */
@@ -106,23 +106,30 @@ int sep_qr_algorithm_f32_dev(sep_handle *h, int64_t n, float *dv, float *ev, flo
                              int32_t *converged, int64_t *iter, int64_t *n_rotations);
 
 /* ---- Batched solve (no counterpart in the reference) --------------------------
- * B independent problems, problem p of even order n[p] stored at dv + off_d[p],
- * ev + off_d[p] (n[p]-1 used) and Q + off_q[p] (n[p] x n[p], column-major, ld n[p]).
- * Device pointers for dv/ev/Q/converged/iters; n/off_d/off_q are HOST arrays.
- * Every problem runs qr_algorithm! semantics (same defaults: tol = eps,
- * maxiter = 100 n[p]) inside one persistent launch, captured in a HIP graph that is
- * replayed for an identical batch layout.  max_layers <= 0 lets the autotuner pick
- * the layer depth.  converged[p] (int32) and iters[p] (int64) are written per
- * problem.  Returns SEP_NOT_CONVERGED if any problem did not converge. */
+ * B independent problems, problem p of even order n[p] (2 <= n[p] <= 8192) stored at
+ * dv + off_d[p], ev + off_d[p] (n[p]-1 used) and Q + off_q[p] (n[p] x n[p], column-major,
+ * ld n[p]).  Device pointers for dv/ev/Q/converged/iters (converged/iters may be NULL);
+ * n/off_d/off_q are HOST arrays.  Every problem runs qr_algorithm! semantics (tol = eps,
+ * maxiter = 100 n[p]): rounds of one block of sweeps per unfinished problem (one
+ * wavefront each) followed by the block's application to Q, until every problem is done.
+ * The results equal per-problem sep_qr_algorithm_f64 up to the apply's rounding form
+ * (here Rotation2 rmul! arithmetic, bit for bit the layer-major product).
+ * max_layers = 0 takes the autotuned (or default, 32) block length.  converged[p]
+ * (int32) and iters[p] (int64) are written per problem.  Synchronous: returns after the
+ * solve.  Returns SEP_NOT_CONVERGED if any problem did not converge. */
 int sep_qr_batched_f64_dev(sep_handle *h, int64_t B, const int64_t *n, const int64_t *off_d,
                            const int64_t *off_q, double *dv, double *ev, double *Q,
                            int64_t max_layers, int32_t *converged, int64_t *iters);
-/* Autotune the batched kernel's launch parameters for this batch layout (timed on
- * scratch copies; the inputs are not modified).  Writes the chosen configuration
- * index to *choice (may be NULL).  The table lives in the handle. */
+/* Autotune the batched solver for this batch layout: block length (max_layers), fused
+ * rotation depth and rows of Q per lane, each configuration timed on scratch copies (the
+ * inputs are not modified).  Writes the chosen configuration index to *choice (may be
+ * NULL); later solves of the same layout on this handle use it. */
 int sep_tune_batched_f64(sep_handle *h, int64_t B, const int64_t *n, const int64_t *off_d,
                          const int64_t *off_q, const double *dv, const double *ev,
                          int32_t *choice);
+/* Force the batched apply's fused depth and rows per lane (8/1, 8/2 or 4/1) for later
+ * solves on this handle, overriding the autotuner; 0/0 restores it. */
+int sep_set_batched_config(sep_handle *h, int32_t fused_depth, int32_t rows_per_lane);
 
 /* ---- Backward of the eigendecomposition (no counterpart in the reference) -----
  * Given eigenvalues lam (n), eigenvectors Q (n x n, column-major, ldq) of the

@@ -52,7 +52,7 @@ EXPORTS = (
     "sep_bulk_rmul_f64", "sep_bulk_rmul_f32", "sep_bulk_rmul_f64_dev", "sep_bulk_rmul_f32_dev",
     "sep_qr_algorithm_f64", "sep_qr_algorithm_f32", "sep_qr_algorithm_f64_dev",
     "sep_qr_algorithm_f32_dev",
-    "sep_qr_batched_f64_dev", "sep_tune_batched_f64",
+    "sep_qr_batched_f64_dev", "sep_tune_batched_f64", "sep_set_batched_config",
     "sep_eig_backward_f64_dev", "sep_eig_backward_batched_f64_dev",
 )
 
@@ -104,6 +104,7 @@ def _declare(L: C.CDLL) -> None:
             f.argtypes = [_P, _I64, _P, _P, _P, _I64, _I64, ct, _I64, _P, _P, _P]
     L.sep_qr_batched_f64_dev.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _I64, _P, _P]
     L.sep_tune_batched_f64.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P]
+    L.sep_set_batched_config.argtypes = [_P, C.c_int32, C.c_int32]
     L.sep_eig_backward_f64_dev.argtypes = [_P, _I64, _P, _P, _I64, _P, _P, _P, _P]
     L.sep_eig_backward_batched_f64_dev.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P]
     for name in EXPORTS:
@@ -156,6 +157,10 @@ class Handle:
     def set_timing(self, enable: bool) -> None:
         """Bracket every bulk-apply kernel with HIP events (resets the counters)."""
         _check(lib().sep_set_timing(self._h, 1 if enable else 0))
+
+    def set_batched_config(self, fused_depth: int = 0, rows_per_lane: int = 0) -> None:
+        """Force the batched apply's fused depth / rows per lane (8/1, 8/2, 4/1); 0/0 = tuned."""
+        _check(lib().sep_set_batched_config(self._h, int(fused_depth), int(rows_per_lane)))
 
     def get_timing(self) -> Tuple[float, float, int]:
         """(apply-kernel ms, preparation-kernels ms, apply launches) since the last reset."""
@@ -357,7 +362,7 @@ def _host_i64(a) -> np.ndarray:
 
 
 def qr_batched_(ns, off_d, off_q, dv, ev, Q, max_layers: int = 0, handle: Optional[Handle] = None):
-    """Solve B problems in one persistent launch (Float64, device tensors).
+    """Solve B problems (Float64, device tensors) with the round-based batched solver.
 
     Problem p: order ns[p] (even), dv/ev at off_d[p], Q (ns[p]^2, column-major) at off_q[p].
     Returns (converged[B] bool numpy, iters[B] int64 numpy)."""

@@ -1,22 +1,24 @@
-// batched.hip -- thousands of independent qr_algorithm! solves (Float64) in one persistent
-// launch (no counterpart in the reference, which solves one matrix per call).
+// batched.hip -- thousands of independent qr_algorithm! solves (Float64) per call (no
+// counterpart in the reference, which solves one matrix per call).
 //
-// Each wavefront of the persistent grid repeatedly takes the next problem from an atomic
-// work queue (problems ordered largest first) and solves it completely:
-//   * the chase of one block of sweeps is the single-solve device chase
-//     (src/bulge_chasing.jl:166-276 semantics, same code), writing its rotations into a
-//     time-major table in the wave's scratch slot: rotation (i, L) at row i + 2L, so that
-//     the rotations of one wave-order time step are contiguous;
-//   * the table is converted in place to shear coefficients (p, q) = (s/(1+c), -s)
-//     (never-written entries stay (0, 0), the identity);
-//   * the wave applies the block to its problem's Q in wave order, KF layers per pass over
-//     Q, one row of Q per lane, the window of 2 KF columns in VGPRs, three DPP-fed FMAs per
-//     rotation and row, and Q columns and coefficient rows prefetched P steps ahead;
-//   * the outer loop (:166-285) repeats until no sweep runs or maxiter is exceeded.
-// Rows-per-lane is fixed at one (a row group of 64 rows per pass); fused depth KF, the
-// sweep block length (max_layers) and the number of persistent waves are chosen by the
-// autotuner.  The reset of the queue and the persistent kernel are captured in a HIP graph
-// that is replayed for an identical batch layout.
+// The outer loop of qr_algorithm! (src/bulge_chasing.jl:166-285) runs for all problems at
+// once.  One round is two launches over the whole batch:
+//   k_chase_all:  one wavefront per problem runs one block of sweeps with the single
+//                 solve's device chase (chase.hpp, src/bulge_chasing.jl:166-276), in the
+//                 reference's (n-1) x max_layers grid layout; a problem that has finished
+//                 (no sweep in its last block, or maxiter exceeded) returns at once;
+//   k_apply_all:  the block's rotations are applied to each active problem's Q in wave
+//                 order t = i + 2L (src/wave.jl:62-162), KF layers per pass over Q, R rows
+//                 of Q per lane, the window of 2 KF columns per row in VGPRs, and the
+//                 (c, s) of 16 time steps staged in LDS at a time.  Rotations beyond a
+//                 column's layer count are read as the identity, so the grid never needs
+//                 an identity fill.  The rotation is Rotation2 rmul! (src/simple_givens.jl:
+//                 53-64) operation for operation, so Q equals the layer-major product of
+//                 the C restatement bit for bit.
+// The host replays rounds until a device counter says every problem is done (every
+// round either finishes a problem or advances its iteration count, so the number of
+// rounds is bounded by max maxiter + 2).  The autotuner picks max_layers, the fused depth
+// KF and the rows per lane R.
 #include <algorithm>
 #include <numeric>
 #include <type_traits>
@@ -28,44 +30,49 @@ namespace sep {
 namespace {
 
 constexpr int kLanes = 64;
-constexpr int kPF = 8;  // prefetch distance (time steps)
+constexpr int kStageSteps = 16;  // wave-order time steps staged per LDS fill
+constexpr int kMaxBatchedN = 8192;
 
-#define SEP_BDPP(a, c, b, lane) \
-    "v_fmac_f64_dpp " a ", " c ", " b " row_newbcast:" #lane " row_mask:0xf bank_mask:0xf\n"
+// state word (beyond the chase's own) marking a finished problem
+constexpr int ST_DONE = 10;
+static_assert(ST_DONE < chase::ST_WORDS, "state words");
 
-// Eight rotations in shear form: a += p b; b += q a; a += p b with (p_l, q_l) broadcast from
-// lanes 2l, 2l+1 of each 16-lane row of cf.
-__device__ __forceinline__ void shear8(double& a0, double& b0, double& a1, double& b1, double& a2,
-                                       double& b2, double& a3, double& b3, double& a4,
-                                       double& b4, double& a5, double& b5, double& a6,
-                                       double& b6, double& a7, double& b7, double cf)
+struct BProb {
+    int n;
+    int pad;
+    int64_t off_d;   // dv / ev
+    int64_t off_q;   // Q (n x n, ld n)
+    int64_t off_lt;  // layer counts (n ints)
+    int64_t off_g;   // grid ((n-1) x max_layers pairs)
+};
+
+struct BWork {
+    int prob;
+    int row0;
+};
+
+// One block of sweeps for every unfinished problem (one wavefront each).
+__global__ void __launch_bounds__(kLanes)
+    k_chase_all(const BProb* __restrict__ probs, double* dv, double* ev, int* lt, double* grid,
+                int64_t* st_all, int ML, int* done_count)
 {
-    asm("s_nop 1\n"
-        SEP_BDPP("%0", "%16", "%1", 0) SEP_BDPP("%2", "%16", "%3", 2)
-        SEP_BDPP("%4", "%16", "%5", 4) SEP_BDPP("%6", "%16", "%7", 6)
-        SEP_BDPP("%8", "%16", "%9", 8) SEP_BDPP("%10", "%16", "%11", 10)
-        SEP_BDPP("%12", "%16", "%13", 12) SEP_BDPP("%14", "%16", "%15", 14)
-        SEP_BDPP("%1", "%16", "%0", 1) SEP_BDPP("%3", "%16", "%2", 3)
-        SEP_BDPP("%5", "%16", "%4", 5) SEP_BDPP("%7", "%16", "%6", 7)
-        SEP_BDPP("%9", "%16", "%8", 9) SEP_BDPP("%11", "%16", "%10", 11)
-        SEP_BDPP("%13", "%16", "%12", 13) SEP_BDPP("%15", "%16", "%14", 15)
-        SEP_BDPP("%0", "%16", "%1", 0) SEP_BDPP("%2", "%16", "%3", 2)
-        SEP_BDPP("%4", "%16", "%5", 4) SEP_BDPP("%6", "%16", "%7", 6)
-        SEP_BDPP("%8", "%16", "%9", 8) SEP_BDPP("%10", "%16", "%11", 10)
-        SEP_BDPP("%12", "%16", "%13", 12) SEP_BDPP("%14", "%16", "%15", 14)
-        : "+v"(a0), "+v"(b0), "+v"(a1), "+v"(b1), "+v"(a2), "+v"(b2), "+v"(a3), "+v"(b3),
-          "+v"(a4), "+v"(b4), "+v"(a5), "+v"(b5), "+v"(a6), "+v"(b6), "+v"(a7), "+v"(b7)
-        : "v"(cf));
-}
-
-template <int W, int s, int l0>
-__device__ __forceinline__ void shear8_at(double (&w)[W], double cf)
-{
-#define BSL(l, o) w[(((s) - 2 * ((l0) + (l)) + (o)) % W + W) % W]
-    shear8(BSL(0, 0), BSL(0, 1), BSL(1, 0), BSL(1, 1), BSL(2, 0), BSL(2, 1), BSL(3, 0),
-           BSL(3, 1), BSL(4, 0), BSL(4, 1), BSL(5, 0), BSL(5, 1), BSL(6, 0), BSL(6, 1),
-           BSL(7, 0), BSL(7, 1), cf);
-#undef BSL
+    const int p = blockIdx.x;
+    const int lane = threadIdx.x;
+    int64_t* st = st_all + (int64_t)p * chase::ST_WORDS;
+    if (__builtin_amdgcn_readfirstlane((int)st[ST_DONE])) return;
+    const BProb pr = probs[p];
+    chase::Problem<double> P{dv + pr.off_d, ev + pr.off_d, lt + pr.off_lt, grid + 2 * pr.off_g,
+                             pr.n, ML, (int64_t)(pr.n - 1), 0};
+    const double tol = 2.220446049250313e-16;  // eps(Float64)
+    chase::chase_block(P, tol, 100 * (int64_t)pr.n, st, lane);
+    __threadfence_block();
+    if (lane == 0) {
+        const bool done = st[chase::ST_EARLY] != 0 || st[chase::ST_DID] == 0;
+        if (done) {
+            st[ST_DONE] = 1;
+            atomicAdd(done_count, 1);
+        }
+    }
 }
 
 template <int N, int i0>
@@ -80,187 +87,124 @@ struct Unroll {
     }
 };
 
-// Scratch of one persistent wave.
-struct Slot {
-    double* tab;     // time-major table, tab_rows x MW pairs
-    int* lt;         // layer_thickness, nmax
-    int64_t* st;     // chase state words
-};
+#pragma clang fp contract(off)
 
-// One pass of KF layers [L0, L0 + KF) of the block over Q (n x n, ld n), 64 rows at a time.
-// MW is the table row width in pairs (a multiple of KF).
-template <int KF>
-__device__ void apply_pass(double* __restrict__ Q, int n, const double* __restrict__ tab, int MW,
-                           int L0, int lane)
+// Apply the block's rotations to Q for rows [row0, row0 + 64 R) of one problem.
+template <int KF, int R>
+__global__ void __launch_bounds__(kLanes)
+    k_apply_all(const BWork* __restrict__ work, const BProb* __restrict__ probs,
+                const int* __restrict__ lt, const double* __restrict__ grid,
+                const int64_t* __restrict__ st_all, double* __restrict__ Q)
 {
-    constexpr int W = 2 * KF, NRW = KF / 8, P = kPF;
-    static_assert(W % P == 0, "prefetch ring must divide the window period");
-    const int cl = lane & 15;
-    const int T = n + 2 * KF - 2;
-    const int nblk = (T + W - 1) / W;
-    // coefficient row of time tau: table row tau + 2 L0, pairs L0 .. L0 + KF - 1
-    const double* __restrict__ cbase = tab + (int64_t)(2 * L0) * MW * 2 + 2 * L0 + cl;
-    const int64_t crow = (int64_t)MW * 2;
-    for (int r0 = 0; r0 < n; r0 += kLanes) {
-        const int row = min(r0 + lane, n - 1);
-        double* __restrict__ Qr = Q + row;
-        double win[W];
+    constexpr int W = 2 * KF;
+    constexpr int SB = kStageSteps;
+    static_assert(SB % W == 0, "staged steps must be a multiple of the window");
+    __shared__ double2 stg[SB][KF];
+
+    const BWork wk = work[blockIdx.x];
+    const int64_t* st = st_all + (int64_t)wk.prob * chase::ST_WORDS;
+    // nothing to apply: finished (no sweep, or maxiter exceeded: the reference returns
+    // without applying that block, src/bulge_chasing.jl:180-182)
+    if (__builtin_amdgcn_readfirstlane((int)st[ST_DONE])) return;
+    const int depth = __builtin_amdgcn_readfirstlane((int)st[chase::ST_MAXLAYER]);
+    const BProb pr = probs[wk.prob];
+    const int n = pr.n;
+    const int lane = threadIdx.x;
+    const int* __restrict__ ltp = lt + pr.off_lt;
+    const double* __restrict__ g = grid + 2 * pr.off_g;
+    double* __restrict__ Qp = Q + pr.off_q;
+    int rowi[R];
+    bool act[R];
 #pragma unroll
-        for (int w = 0; w < W; ++w) win[w] = 0.0;
-        win[0] = Qr[0];
-        double cq[P][NRW], qi[P];
+    for (int r = 0; r < R; ++r) {
+        const int row = wk.row0 + r * kLanes + lane;
+        act[r] = row < n;
+        rowi[r] = min(row, n - 1);
+    }
+    const int Tn = n + 2 * KF - 2;  // time steps of one pass, the last one only stores
+    for (int L0 = 0; L0 < depth; L0 += KF) {
+        double win[R][W];
 #pragma unroll
-        for (int s = 0; s < P; ++s) {
+        for (int r = 0; r < R; ++r) {
 #pragma unroll
-            for (int r = 0; r < NRW; ++r) cq[s][r] = cbase[s * crow + 16 * r];
-            qi[s] = Qr[(int64_t)min(s + 1, n - 1) * n];
+            for (int w = 0; w < W; ++w) win[r][w] = 0.0;
+            win[r][0] = act[r] ? Qp[rowi[r]] : 0.0;
         }
-        for (int b = 0; b < nblk; ++b) {
-            const int tau0 = b * W;
-            Unroll<W, 0>::run([&](auto sc) {
+        for (int t0 = 0; t0 < Tn; t0 += SB) {
+            __syncthreads();  // the previous fill has been read
+            for (int q = lane; q < SB * KF; q += kLanes) {
+                const int s = q / KF, l = q % KF;
+                const int i = t0 + s - 2 * l, L = L0 + l;
+                double2 v = make_double2(1.0, 0.0);
+                if (i >= 0 && i <= n - 2 && L < depth && L < ltp[i])
+                    v = *reinterpret_cast<const double2*>(g + 2 * (i + (int64_t)(n - 1) * L));
+                stg[s][l] = v;
+            }
+            __syncthreads();
+            Unroll<SB, 0>::run([&](auto sc) {
                 constexpr int s = decltype(sc)::value;
-                constexpr int k = s % P;
-                const int tau = tau0 + s;
-                win[(s + 1) % W] = qi[k];
-                shear8_at<W, s, 0>(win, cq[k][0]);
-                if constexpr (NRW > 1) shear8_at<W, s, 8>(win, cq[k][1]);
-                // refill this ring slot with time tau + P
+                const int t = t0 + s;
+                if (t < Tn) {
+                    const int jn = t + 1;  // column entering the window
 #pragma unroll
-                for (int r = 0; r < NRW; ++r) cq[k][r] = cbase[(int64_t)(tau + P) * crow + 16 * r];
-                qi[k] = Qr[(int64_t)min(tau + P + 1, n - 1) * n];
-                const int col = tau - 2 * KF + 2;
-                if (col >= 0 && col < n) Qr[(int64_t)col * n] = win[(s + 2) % W];
+                    for (int r = 0; r < R; ++r)
+                        win[r][(s + 1) % W] =
+                            (act[r] && jn < n) ? Qp[rowi[r] + (int64_t)jn * n] : 0.0;
+                    Unroll<KF, 0>::run([&](auto lc) {
+                        constexpr int l = decltype(lc)::value;
+                        constexpr int ia = ((s - 2 * l) % W + W) % W;
+                        constexpr int ib = (ia + 1) % W;
+                        const double2 cs = stg[s][l];
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            const double x = win[r][ia], y = win[r][ib];
+                            win[r][ia] = x * cs.x + y * cs.y;
+                            win[r][ib] = x * -cs.y + y * cs.x;
+                        }
+                    });
+                    const int jo = t - 2 * KF + 2;  // column leaving the window
+                    if (jo >= 0 && jo < n) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r)
+                            if (act[r]) Qp[rowi[r] + (int64_t)jo * n] = win[r][(s + 2) % W];
+                    }
+                }
             });
         }
     }
 }
 
-__device__ __forceinline__ int wave_bcast(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-// qr_algorithm! on one problem by one wave (src/bulge_chasing.jl:146-288).  ML is the
-// sweep block length (max_layers), MW >= ML the table row width in pairs.
-template <int KF>
-__device__ void solve_one(double* dv, double* ev, double* Q, int n, int ML, int MW, int tab_rows,
-                          const Slot& sl, int32_t* conv_out, int64_t* iter_out, int lane)
-{
-    using chase::ST_DID;
-    using chase::ST_EARLY;
-    using chase::ST_ITER;
-    using chase::ST_LBEA;
-    using chase::ST_MAXLAYER;
-    using chase::ST_TO;
-    const double tol = 2.220446049250313e-16;  // eps(Float64)
-    const int64_t maxiter = 100 * (int64_t)n;
-    if (lane == 0) {
-        sl.st[ST_TO] = n;
-        sl.st[ST_LBEA] = n;
-        sl.st[ST_ITER] = 0;
-    }
-    __threadfence_block();
-    chase::Problem<double> P{dv, ev, sl.lt, sl.tab, n, ML, 0, MW};
-    int conv = 1;
-    const int64_t tab_pairs = (int64_t)tab_rows * MW;
-    for (;;) {
-        // unwritten table entries must read as (0, 0), the identity in shear form
-        for (int64_t q = lane; q < tab_pairs; q += kLanes)
-            *reinterpret_cast<double2*>(sl.tab + 2 * q) = make_double2(0.0, 0.0);
-        __threadfence_block();
-        chase::chase_block(P, tol, maxiter, sl.st, lane);
-        __threadfence_block();
-        const int early = wave_bcast((int)sl.st[ST_EARLY]);
-        const int did = wave_bcast((int)sl.st[ST_DID]);
-        if (early) {
-            conv = 0;
-            break;
-        }
-        if (!did) break;
-        const int depth = wave_bcast((int)sl.st[ST_MAXLAYER]);
-        // (c, s) -> (p, q) over the rows the block can have written
-        const int rows = n + 2 * depth;
-        for (int q = lane; q < rows * depth; q += kLanes) {
-            const int t = q / depth, L = q % depth;
-            double2* e = reinterpret_cast<double2*>(sl.tab + 2 * ((int64_t)t * MW + L));
-            const double2 cs = *e;
-            *e = make_double2(cs.y / (1.0 + cs.x), -cs.y);
-        }
-        __threadfence_block();
-        for (int L0 = 0; L0 < depth; L0 += KF) apply_pass<KF>(Q, n, sl.tab, MW, L0, lane);
-        __threadfence_block();
-    }
-    if (lane == 0) {
-        *conv_out = conv;
-        *iter_out = sl.st[ST_ITER];
-    }
-}
-
-template <int KF>
-__global__ void __launch_bounds__(64) k_batched(int B, const int* __restrict__ order,
-                                                const int* __restrict__ ns,
-                                                const int64_t* __restrict__ off_d,
-                                                const int64_t* __restrict__ off_q, double* dv,
-                                                double* ev, double* Q, int ML, int MW, int tab_rows,
-                                                double* tab, int* lt, int64_t* st, int nmax,
-                                                int* queue, int32_t* conv, int64_t* iters)
-{
-    const int lane = threadIdx.x;
-    const int slot = blockIdx.x;
-    Slot sl{tab + (int64_t)slot * tab_rows * MW * 2, lt + (int64_t)slot * nmax,
-            st + (int64_t)slot * chase::ST_WORDS};
-    for (;;) {
-        int p = 0;
-        if (lane == 0) p = atomicAdd(queue, 1);
-        p = __shfl(p, 0, kLanes);
-        p = wave_bcast(p);
-        if (p >= B) break;
-        const int prob = order[p];
-        const int n = ns[prob];
-        solve_one<KF>(dv + off_d[prob], ev + off_d[prob], Q + off_q[prob], n, ML, MW, tab_rows,
-                      sl, conv + prob, iters + prob, lane);
-    }
-}
-
-__global__ void k_reset_queue(int* q) { *q = 0; }
-
-// The persistent kernel below is switched off: a GPU run of it did not complete, and no
-// mistake that explains it has been found yet (the same device code, executed on the host
-// by a 64-lane lockstep emulation, finishes and matches the C restatement on that input).
-// Both entry points validate their arguments and then return SEP_ERR_UNSUPPORTED.
-constexpr bool kBatchedEnabled = false;
-constexpr const char* kBatchedDisabledWhy =
-    "the persistent batched solver is disabled in this build (a GPU run of it did not "
-    "complete and the cause is not identified); solve problems one at a time";
+#pragma clang fp contract(on)
 
 struct Config {
-    int KF, ML, NW;
+    int ML, KF, R;
 };
 constexpr Config kConfigs[] = {
-    {16, 32, 4096}, {16, 64, 4096}, {16, 16, 4096}, {8, 32, 4096},  {8, 16, 4096},
-    {16, 32, 2048}, {16, 64, 2048}, {8, 32, 2048},  {16, 32, 8192}, {8, 16, 8192},
+    {32, 8, 1}, {32, 8, 2}, {64, 8, 1}, {64, 8, 2}, {16, 8, 1}, {32, 4, 1}, {16, 4, 1},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 
-int tab_rows_for(int nmax, int ML, int KF) { return nmax + 2 * ML + 4 * KF + kPF + 8; }
-// table row width: max_layers rounded up to the fused depth (pairs beyond ML stay (0, 0))
-int row_width_for(int ML, int KF) { return (ML + KF - 1) / KF * KF; }
-
 struct Layout {
-    std::vector<int> order, ns32;
+    std::vector<BProb> probs;
+    std::vector<BWork> work;
     int nmax = 0;
+    int64_t tot_lt = 0, tot_g = 0;
 };
 
-int check_layout(int64_t B, const int64_t* n, const int64_t* off_d, const int64_t* off_q,
-                 Layout& L)
+int check_batch(int64_t B, const int64_t* n, const int64_t* off_d, const int64_t* off_q)
 {
     if (B < 0 || (B > 0 && (!n || !off_d || !off_q))) {
         set_error("batched: invalid batch description");
         return SEP_ERR_ARG;
     }
-    L.ns32.resize(B);
-    L.order.resize(B);
+    if (B > (int64_t)1 << 30) {
+        set_error("batched: too many problems (%lld)", (long long)B);
+        return SEP_ERR_ARG;
+    }
     for (int64_t p = 0; p < B; ++p) {
-        if (n[p] < 2 || n[p] > 8192 || off_d[p] < 0 || off_q[p] < 0) {
-            set_error("batched: problem %lld has order %lld (2 <= n <= 8192) or bad offsets",
-                      (long long)p, (long long)n[p]);
+        if (n[p] < 2 || n[p] > kMaxBatchedN || off_d[p] < 0 || off_q[p] < 0) {
+            set_error("batched: problem %lld has order %lld (2 <= n <= %d) or bad offsets",
+                      (long long)p, (long long)n[p], kMaxBatchedN);
             return SEP_ERR_ARG;
         }
         if (n[p] % 2) {
@@ -268,72 +212,125 @@ int check_layout(int64_t B, const int64_t* n, const int64_t* off_d, const int64_
                       (long long)p, (long long)n[p]);
             return SEP_ERR_ODD_N;
         }
-        L.ns32[p] = (int)n[p];
+    }
+    return SEP_OK;
+}
+
+void build_layout(int64_t B, const int64_t* n, const int64_t* off_d, const int64_t* off_q,
+                  const Config& cfg, Layout& L)
+{
+    L.probs.resize(B);
+    L.work.clear();
+    L.nmax = 0;
+    L.tot_lt = L.tot_g = 0;
+    for (int64_t p = 0; p < B; ++p) {
+        BProb& q = L.probs[p];
+        q.n = (int)n[p];
+        q.pad = 0;
+        q.off_d = off_d[p];
+        q.off_q = off_q[p];
+        q.off_lt = L.tot_lt;
+        q.off_g = L.tot_g;
+        L.tot_lt += n[p];
+        L.tot_g += (n[p] - 1) * cfg.ML;
         L.nmax = std::max(L.nmax, (int)n[p]);
     }
-    std::iota(L.order.begin(), L.order.end(), 0);
-    // longest-processing-time first: the apply costs ~n^3, the chase ~n^2
-    std::stable_sort(L.order.begin(), L.order.end(),
-                     [&](int a, int b) { return L.ns32[a] > L.ns32[b]; });
-    return SEP_OK;
+    // largest problems first: their apply is the longest
+    std::vector<int> order(B);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return n[a] > n[b]; });
+    const int rows = kLanes * cfg.R;
+    for (int p : order)
+        for (int r0 = 0; r0 < (int)n[p]; r0 += rows) L.work.push_back(BWork{p, r0});
 }
 
-// Upload the layout and scratch for `cfg`; returns device pointers through the handle.
-int prepare(sep_handle* h, int64_t B, const int64_t* off_d, const int64_t* off_q,
-            const Layout& L, const Config& cfg, int& tab_rows)
+template <int KF, int R>
+void launch_apply(int nwork, hipStream_t s, const BWork* w, const BProb* p, const int* lt,
+                  const double* g, const int64_t* st, double* Q)
 {
-    tab_rows = tab_rows_for(L.nmax, cfg.ML, cfg.KF);
-    const size_t meta = sizeof(int) * 2 * B + sizeof(int64_t) * 2 * B + 64;
-    if (int rc = h->batch_meta.reserve(meta)) return rc;
-    char* m = static_cast<char*>(h->batch_meta.p);
-    int* d_order = reinterpret_cast<int*>(m);
-    int* d_ns = d_order + B;
-    int64_t* d_offd = reinterpret_cast<int64_t*>(m + ((sizeof(int) * 2 * B + 15) / 16) * 16);
-    int64_t* d_offq = d_offd + B;
-    int* d_queue = reinterpret_cast<int*>(d_offq + B);
-    (void)d_queue;
-    SEP_HIP(hipMemcpyAsync(d_order, L.order.data(), sizeof(int) * B, hipMemcpyHostToDevice,
-                           h->stream));
-    SEP_HIP(hipMemcpyAsync(d_ns, L.ns32.data(), sizeof(int) * B, hipMemcpyHostToDevice, h->stream));
-    SEP_HIP(hipMemcpyAsync(d_offd, off_d, sizeof(int64_t) * B, hipMemcpyHostToDevice, h->stream));
-    SEP_HIP(hipMemcpyAsync(d_offq, off_q, sizeof(int64_t) * B, hipMemcpyHostToDevice, h->stream));
-    const int NW = (int)std::min<int64_t>(cfg.NW, std::max<int64_t>(B, 1));
-    const size_t tab_bytes =
-        sizeof(double) * 2 * (size_t)tab_rows * row_width_for(cfg.ML, cfg.KF) * NW;
-    const size_t lt_bytes = sizeof(int) * (size_t)L.nmax * NW;
-    const size_t st_bytes = sizeof(int64_t) * chase::ST_WORDS * NW;
-    if (int rc = h->batch_scratch.reserve(tab_bytes + lt_bytes + st_bytes + 256)) return rc;
-    return SEP_OK;
+    hipLaunchKernelGGL((k_apply_all<KF, R>), dim3(nwork), dim3(kLanes), 0, s, w, p, lt, g, st, Q);
 }
 
-int launch(sep_handle* h, hipStream_t s, int64_t B, double* dv, double* ev, double* Q,
-           const Layout& L, const Config& cfg, int tab_rows, int32_t* conv, int64_t* iters)
+// The whole batched solve on the handle's stream; returns after completion.
+int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d,
+                const int64_t* off_q, double* dv, double* ev, double* Q, const Config& cfg,
+                int32_t* converged, int64_t* iters)
 {
+    Layout L;
+    build_layout(B, n, off_d, off_q, cfg, L);
+    const size_t probs_b = sizeof(BProb) * (size_t)B;
+    const size_t work_b = sizeof(BWork) * L.work.size();
+    const size_t st_b = sizeof(int64_t) * chase::ST_WORDS * (size_t)B;
+    auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+    if (int rc = h->batch_meta.reserve(al(probs_b) + al(work_b) + al(st_b) + 256)) return rc;
+    if (int rc = h->batch_scratch.reserve(al(sizeof(int) * (size_t)L.tot_lt) +
+                                          sizeof(double) * 2 * (size_t)L.tot_g + 256))
+        return rc;
     char* m = static_cast<char*>(h->batch_meta.p);
-    int* d_order = reinterpret_cast<int*>(m);
-    int* d_ns = d_order + B;
-    int64_t* d_offd = reinterpret_cast<int64_t*>(m + ((sizeof(int) * 2 * B + 15) / 16) * 16);
-    int64_t* d_offq = d_offd + B;
-    int* d_queue = reinterpret_cast<int*>(d_offq + B);
-    const int NW = (int)std::min<int64_t>(cfg.NW, std::max<int64_t>(B, 1));
-    const int MW = row_width_for(cfg.ML, cfg.KF);
+    BProb* d_probs = reinterpret_cast<BProb*>(m);
+    BWork* d_work = reinterpret_cast<BWork*>(m + al(probs_b));
+    int64_t* d_st = reinterpret_cast<int64_t*>(m + al(probs_b) + al(work_b));
+    int* d_count = reinterpret_cast<int*>(m + al(probs_b) + al(work_b) + al(st_b));
     char* sc = static_cast<char*>(h->batch_scratch.p);
-    double* tab = reinterpret_cast<double*>(sc);
-    int* lt = reinterpret_cast<int*>(sc + sizeof(double) * 2 * (size_t)tab_rows * MW * NW);
-    int64_t* st = reinterpret_cast<int64_t*>(
-        reinterpret_cast<char*>(lt) + ((sizeof(int) * (size_t)L.nmax * NW + 15) / 16) * 16);
-    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, s, d_queue);
-    SEP_CHECK_LAUNCH();
-    if (cfg.KF == 8)
-        hipLaunchKernelGGL((k_batched<8>), dim3(NW), dim3(64), 0, s, (int)B, d_order, d_ns,
-                           d_offd, d_offq, dv, ev, Q, cfg.ML, MW, tab_rows, tab, lt, st, L.nmax,
-                           d_queue, conv, iters);
-    else
-        hipLaunchKernelGGL((k_batched<16>), dim3(NW), dim3(64), 0, s, (int)B, d_order, d_ns,
-                           d_offd, d_offq, dv, ev, Q, cfg.ML, MW, tab_rows, tab, lt, st, L.nmax,
-                           d_queue, conv, iters);
-    SEP_CHECK_LAUNCH();
-    return SEP_OK;
+    int* d_lt = reinterpret_cast<int*>(sc);
+    double* d_grid = reinterpret_cast<double*>(sc + al(sizeof(int) * (size_t)L.tot_lt));
+
+    // host staging lives in the handle; every copy below completes before it is reused
+    h->batch_host_st.assign((size_t)chase::ST_WORDS * B, 0);
+    for (int64_t p = 0; p < B; ++p) {
+        int64_t* s = &h->batch_host_st[(size_t)p * chase::ST_WORDS];
+        s[chase::ST_TO] = n[p];
+        s[chase::ST_LBEA] = n[p];
+        s[chase::ST_ITER] = 0;
+    }
+    int done = 0;
+    SEP_HIP(hipMemcpyAsync(d_probs, L.probs.data(), probs_b, hipMemcpyHostToDevice, h->stream));
+    SEP_HIP(hipMemcpyAsync(d_work, L.work.data(), work_b, hipMemcpyHostToDevice, h->stream));
+    SEP_HIP(hipMemcpyAsync(d_st, h->batch_host_st.data(), st_b, hipMemcpyHostToDevice, h->stream));
+    SEP_HIP(hipMemcpyAsync(d_count, &done, sizeof(int), hipMemcpyHostToDevice, h->stream));
+    SEP_HIP(hipStreamSynchronize(h->stream));
+
+    const int nwork = (int)L.work.size();
+    const int64_t max_rounds = 100 * (int64_t)L.nmax + 3;
+    int64_t round = 0;
+    for (; round < max_rounds && done < B; ++round) {
+        hipLaunchKernelGGL(k_chase_all, dim3((unsigned)B), dim3(kLanes), 0, h->stream, d_probs,
+                           dv, ev, d_lt, d_grid, d_st, cfg.ML, d_count);
+        SEP_CHECK_LAUNCH();
+        if (cfg.KF == 8 && cfg.R == 2)
+            launch_apply<8, 2>(nwork, h->stream, d_work, d_probs, d_lt, d_grid, d_st, Q);
+        else if (cfg.KF == 8)
+            launch_apply<8, 1>(nwork, h->stream, d_work, d_probs, d_lt, d_grid, d_st, Q);
+        else
+            launch_apply<4, 1>(nwork, h->stream, d_work, d_probs, d_lt, d_grid, d_st, Q);
+        SEP_CHECK_LAUNCH();
+        SEP_HIP(hipMemcpyAsync(&done, d_count, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        SEP_HIP(hipStreamSynchronize(h->stream));
+    }
+    if (done < B) {
+        set_error("batched: %d of %lld problems unfinished after %lld rounds", (int)(B - done),
+                  (long long)B, (long long)round);
+        return SEP_ERR_HIP;
+    }
+    SEP_HIP(hipMemcpyAsync(h->batch_host_st.data(), d_st, st_b, hipMemcpyDeviceToHost, h->stream));
+    SEP_HIP(hipStreamSynchronize(h->stream));
+    std::vector<int32_t> conv((size_t)B);
+    std::vector<int64_t> its((size_t)B);
+    int all = 1;
+    for (int64_t p = 0; p < B; ++p) {
+        const int64_t* s = &h->batch_host_st[(size_t)p * chase::ST_WORDS];
+        conv[p] = s[chase::ST_EARLY] ? 0 : 1;
+        its[p] = s[chase::ST_ITER];
+        all &= conv[p];
+    }
+    if (converged)
+        SEP_HIP(hipMemcpyAsync(converged, conv.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice,
+                               h->stream));
+    if (iters)
+        SEP_HIP(hipMemcpyAsync(iters, its.data(), sizeof(int64_t) * B, hipMemcpyHostToDevice,
+                               h->stream));
+    SEP_HIP(hipStreamSynchronize(h->stream));
+    return all ? SEP_OK : SEP_NOT_CONVERGED;
 }
 
 std::vector<int64_t> layout_key(int64_t B, const int64_t* n, const int64_t* off_d,
@@ -348,31 +345,76 @@ std::vector<int64_t> layout_key(int64_t B, const int64_t* n, const int64_t* off_
     return k;
 }
 
-__global__ void k_identity_batch(double* Q, const int* ns, const int64_t* off_q, int B)
+__global__ void k_identity_batch(double* Q, const int64_t* __restrict__ nq,
+                                 const int64_t* __restrict__ off_q, int B)
 {
     const int p = blockIdx.y;
     if (p >= B) return;
-    const int n = ns[p];
+    const int64_t n = nq[p];
     double* q = Q + off_q[p];
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < (int64_t)n * n;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * n;
          e += (int64_t)gridDim.x * blockDim.x)
         q[e] = (e % (n + 1) == 0) ? 1.0 : 0.0;
 }
 
 }  // namespace
 
+int qr_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d,
+                   const int64_t* off_q, double* dv, double* ev, double* Q, int64_t max_layers,
+                   int32_t* converged, int64_t* iters)
+{
+    if (int rc = check_batch(B, n, off_d, off_q)) return rc;
+    if (B == 0) return SEP_OK;
+    if (!dv || !ev || !Q) {
+        set_error("batched: null device pointer");
+        return SEP_ERR_ARG;
+    }
+    if (max_layers < 0 || max_layers > 256) {
+        set_error("batched: max_layers=%lld outside [0, 256] (0 = tuned / default)",
+                  (long long)max_layers);
+        return SEP_ERR_BAD_LAYERS;
+    }
+    Config cfg = kConfigs[0];
+    if (h->tune_choice >= 0 && h->tune_key == layout_key(B, n, off_d, off_q))
+        cfg = kConfigs[h->tune_choice];
+    if (h->force_kf > 0) {
+        cfg.KF = h->force_kf;
+        cfg.R = h->force_r;
+    }
+    if (max_layers > 0) cfg.ML = (int)max_layers;
+    return run_batched(h, B, n, off_d, off_q, dv, ev, Q, cfg, converged, iters);
+}
+
+int set_batched_config(sep_handle* h, int fused_depth, int rows_per_lane)
+{
+    if (fused_depth == 0 && rows_per_lane == 0) {
+        h->force_kf = h->force_r = 0;
+        return SEP_OK;
+    }
+    const bool ok = (fused_depth == 8 && (rows_per_lane == 1 || rows_per_lane == 2)) ||
+                    (fused_depth == 4 && rows_per_lane == 1);
+    if (!ok) {
+        set_error("batched: unsupported configuration fused_depth=%d rows_per_lane=%d "
+                  "(supported: 8/1, 8/2, 4/1; 0/0 restores the autotuned choice)",
+                  fused_depth, rows_per_lane);
+        return SEP_ERR_ARG;
+    }
+    h->force_kf = fused_depth;
+    h->force_r = rows_per_lane;
+    return SEP_OK;
+}
+
 int tune_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d,
                      const int64_t* off_q, const double* dv, const double* ev, int32_t* choice)
 {
-    Layout L;
-    if (int rc = check_layout(B, n, off_d, off_q, L)) return rc;
+    if (int rc = check_batch(B, n, off_d, off_q)) return rc;
     if (B == 0) {
         if (choice) *choice = 0;
         return SEP_OK;
     }
-    if (!kBatchedEnabled) {
-        set_error("batched: %s", kBatchedDisabledWhy);
-        return SEP_ERR_UNSUPPORTED;
+    if (!dv || !ev) {
+        set_error("batched: null device pointer");
+        return SEP_ERR_ARG;
     }
     // scratch copies of the inputs (the caller's arrays are not modified)
     int64_t nd = 0, nq = 0;
@@ -380,124 +422,63 @@ int tune_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const int64_t* 
         nd = std::max(nd, off_d[p] + n[p]);
         nq = std::max(nq, off_q[p] + n[p] * n[p]);
     }
-    DevBuf cd, ce, cq, cc, ci, cns, cofq;
-    int rc = SEP_OK;
-    auto fail = [&](int r) {
-        cd.release(); ce.release(); cq.release(); cc.release(); ci.release();
-        cns.release(); cofq.release();
-        return r;
+    DevBuf cd, ce, cq, cmeta;
+    auto release = [&]() {
+        cd.release();
+        ce.release();
+        cq.release();
+        cmeta.release();
     };
+    int rc = SEP_OK;
     if ((rc = cd.reserve(sizeof(double) * nd)) || (rc = ce.reserve(sizeof(double) * nd)) ||
-        (rc = cq.reserve(sizeof(double) * nq)) || (rc = cc.reserve(sizeof(int32_t) * B)) ||
-        (rc = ci.reserve(sizeof(int64_t) * B)) || (rc = cns.reserve(sizeof(int) * B)) ||
-        (rc = cofq.reserve(sizeof(int64_t) * B)))
-        return fail(rc);
-    SEP_HIP(hipMemcpyAsync(cns.p, L.ns32.data(), sizeof(int) * B, hipMemcpyHostToDevice, h->stream));
-    SEP_HIP(hipMemcpyAsync(cofq.p, off_q, sizeof(int64_t) * B, hipMemcpyHostToDevice, h->stream));
-    hipEvent_t e0, e1;
-    SEP_HIP(hipEventCreate(&e0));
-    SEP_HIP(hipEventCreate(&e1));
+        (rc = cq.reserve(sizeof(double) * nq)) || (rc = cmeta.reserve(sizeof(int64_t) * 2 * B))) {
+        release();
+        return rc;
+    }
+    int64_t* d_n = cmeta.as<int64_t>();
+    int64_t* d_offq = d_n + B;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     float best = 1e30f;
     int best_i = 0;
-    for (int c = 0; c < kNumConfigs && rc == SEP_OK; ++c) {
-        int tab_rows = 0;
-        if ((rc = prepare(h, B, off_d, off_q, L, kConfigs[c], tab_rows))) break;
-        SEP_HIP(hipMemcpyAsync(cd.p, dv, sizeof(double) * nd, hipMemcpyDeviceToDevice, h->stream));
-        SEP_HIP(hipMemcpyAsync(ce.p, ev, sizeof(double) * nd, hipMemcpyDeviceToDevice, h->stream));
+    hipError_t he = hipMemcpyAsync(d_n, n, sizeof(int64_t) * B, hipMemcpyHostToDevice, h->stream);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(d_offq, off_q, sizeof(int64_t) * B, hipMemcpyHostToDevice, h->stream);
+    if (he == hipSuccess) he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    for (int c = 0; c < kNumConfigs && he == hipSuccess && rc == SEP_OK; ++c) {
+        he = hipMemcpyAsync(cd.p, dv, sizeof(double) * nd, hipMemcpyDeviceToDevice, h->stream);
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(ce.p, ev, sizeof(double) * nd, hipMemcpyDeviceToDevice, h->stream);
+        if (he != hipSuccess) break;
         hipLaunchKernelGGL(k_identity_batch, dim3(64, (unsigned)B), dim3(256), 0, h->stream,
-                           cq.as<double>(), cns.as<int>(), cofq.as<int64_t>(), (int)B);
-        SEP_CHECK_LAUNCH();
-        SEP_HIP(hipEventRecord(e0, h->stream));
-        if ((rc = launch(h, h->stream, B, cd.as<double>(), ce.as<double>(), cq.as<double>(), L,
-                         kConfigs[c], tab_rows, cc.as<int32_t>(), ci.as<int64_t>())))
-            break;
-        SEP_HIP(hipEventRecord(e1, h->stream));
-        SEP_HIP(hipEventSynchronize(e1));
+                           cq.as<double>(), d_n, d_offq, (int)B);
+        if ((he = hipGetLastError()) != hipSuccess) break;
+        if ((he = hipEventRecord(e0, h->stream)) != hipSuccess) break;
+        rc = run_batched(h, B, n, off_d, off_q, cd.as<double>(), ce.as<double>(), cq.as<double>(),
+                         kConfigs[c], nullptr, nullptr);
+        if (rc == SEP_NOT_CONVERGED) rc = SEP_OK;
+        if (rc) break;
+        if ((he = hipEventRecord(e1, h->stream)) != hipSuccess) break;
+        if ((he = hipEventSynchronize(e1)) != hipSuccess) break;
         float ms = 0;
-        SEP_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if ((he = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
         if (ms < best) {
             best = ms;
             best_i = c;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    fail(SEP_OK);
-    // the tuning runs may have re-laid the scratch the cached graph points into
-    if (h->batched_exec) {
-        (void)hipGraphExecDestroy(h->batched_exec);
-        h->batched_exec = nullptr;
-        h->batched_key.clear();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(h->stream);
+    release();
+    if (he != hipSuccess) {
+        set_error("batched tuner: %s", hipGetErrorString(he));
+        return SEP_ERR_HIP;
     }
     if (rc) return rc;
     h->tune_key = layout_key(B, n, off_d, off_q);
     h->tune_choice = best_i;
     if (choice) *choice = best_i;
-    return SEP_OK;
-}
-
-int qr_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d,
-                   const int64_t* off_q, double* dv, double* ev, double* Q, int64_t max_layers,
-                   int32_t* converged, int64_t* iters)
-{
-    Layout L;
-    if (int rc = check_layout(B, n, off_d, off_q, L)) return rc;
-    if (B == 0) return SEP_OK;
-    if (!dv || !ev || !Q || !converged || !iters) {
-        set_error("batched: null device pointer");
-        return SEP_ERR_ARG;
-    }
-    if (max_layers > 256) {
-        set_error("batched: max_layers=%lld above 256", (long long)max_layers);
-        return SEP_ERR_BAD_LAYERS;
-    }
-    if (!kBatchedEnabled) {
-        set_error("batched: %s", kBatchedDisabledWhy);
-        return SEP_ERR_UNSUPPORTED;
-    }
-    std::vector<int64_t> key = layout_key(B, n, off_d, off_q);
-    Config cfg = kConfigs[0];
-    if (h->tune_choice >= 0 && h->tune_key == key) cfg = kConfigs[h->tune_choice];
-    if (max_layers > 0) cfg.ML = (int)max_layers;
-    // graph cache: the queue reset and the persistent kernel, per layout / pointers / config
-    key.push_back(reinterpret_cast<int64_t>(dv));
-    key.push_back(reinterpret_cast<int64_t>(ev));
-    key.push_back(reinterpret_cast<int64_t>(Q));
-    key.push_back(reinterpret_cast<int64_t>(converged));
-    key.push_back(reinterpret_cast<int64_t>(iters));
-    key.push_back(cfg.KF);
-    key.push_back(cfg.ML);
-    key.push_back(cfg.NW);
-    if (!(h->batched_exec && h->batched_key == key)) {
-        if (h->batched_exec) {
-            (void)hipGraphExecDestroy(h->batched_exec);
-            h->batched_exec = nullptr;
-        }
-        int tab_rows = 0;
-        if (int rc = prepare(h, B, off_d, off_q, L, cfg, tab_rows)) return rc;
-        SEP_HIP(hipStreamSynchronize(h->stream));
-        hipGraph_t g = nullptr;
-        SEP_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        int rc = launch(h, h->stream, B, dv, ev, Q, L, cfg, tab_rows, converged, iters);
-        hipError_t ce = hipStreamEndCapture(h->stream, &g);
-        if (rc) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc;
-        }
-        if (ce != hipSuccess) {
-            set_error("batched: graph capture failed: %s", hipGetErrorString(ce));
-            return SEP_ERR_HIP;
-        }
-        hipError_t ie = hipGraphInstantiate(&h->batched_exec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (ie != hipSuccess) {
-            h->batched_exec = nullptr;
-            set_error("batched: graph instantiate failed: %s", hipGetErrorString(ie));
-            return SEP_ERR_HIP;
-        }
-        h->batched_key = key;
-    }
-    SEP_HIP(hipGraphLaunch(h->batched_exec, h->stream));
     return SEP_OK;
 }
 

@@ -212,7 +212,7 @@ const char* sep_last_error(void) { return sep::g_err; }
 const char* sep_version(void)
 {
     return "libsep_amd 0.1 gfx950 (systolic wave-order Givens apply, device Wilkinson chase, "
-           "batched persistent solver, FP64-MFMA backward)";
+           "batched solver, FP64-MFMA backward)";
 }
 
 int sep_create(sep_handle** out, int device)
@@ -264,7 +264,6 @@ int sep_destroy(sep_handle* h)
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->chase_stream) (void)hipStreamSynchronize(h->chase_stream);
-    if (h->batched_exec) (void)hipGraphExecDestroy(h->batched_exec);
     h->host_io_A.release();
     h->host_io_cs.release();
     h->host_io_d.release();
@@ -434,6 +433,15 @@ int sep_tune_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const int64
     }
     SEP_HIP(hipSetDevice(h->device));
     return tune_batched_f64(h, B, n, off_d, off_q, dv, ev, choice);
+}
+
+int sep_set_batched_config(sep_handle* h, int32_t fused_depth, int32_t rows_per_lane)
+{
+    if (!h) {
+        set_error("sep_set_batched_config: null handle");
+        return SEP_ERR_ARG;
+    }
+    return set_batched_config(h, fused_depth, rows_per_lane);
 }
 
 int sep_eig_backward_f64_dev(sep_handle* h, int64_t n, const double* lam, const double* Q,

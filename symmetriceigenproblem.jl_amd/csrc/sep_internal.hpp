@@ -89,11 +89,11 @@ struct sep_handle {
     sep::DevBuf bwd_meta, bwd_M, bwd_part;
     std::vector<char> bwd_meta_host;
     void* bwd_meta_dev = nullptr;
-    // batched graph cache
-    hipGraphExec_t batched_exec = nullptr;
-    std::vector<int64_t> batched_key;
+    // batched solver: host copy of the per-problem state words, autotuner choice
+    std::vector<int64_t> batch_host_st;
     std::vector<int64_t> tune_key;
     int tune_choice = -1;
+    int force_kf = 0, force_r = 0;  // sep_set_batched_config (0: autotuned / default)
     bool force_sgpr_apply = false;  // SEP_APPLY=sgpr selects the scalar-fed Float64 kernel
     int apply_geom = 0;             // SEP_APPLY=deep selects the 16-wave, 4-layer geometry
     // kernel timing (sep_set_timing): event pairs recorded around apply / prep kernels
@@ -135,6 +135,7 @@ int qr_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const int64_t* of
                    int32_t* converged, int64_t* iters);
 int tune_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d,
                      const int64_t* off_q, const double* dv, const double* ev, int32_t* choice);
+int set_batched_config(sep_handle* h, int fused_depth, int rows_per_lane);
 
 // backward.hip
 int eig_backward_f64(sep_handle* h, int64_t n, const double* lam, const double* Q, int64_t ldq,

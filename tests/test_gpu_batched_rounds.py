@@ -1,0 +1,122 @@
+"""GPU parity of the round-based batched solver (no counterpart in the reference).
+
+Every problem of a batch runs qr_algorithm! semantics (src/bulge_chasing.jl:146-288) with
+the single solve's device chase, and its block of rotations is applied with Rotation2
+rmul! arithmetic (src/simple_givens.jl:53-64) in wave order, which equals the layer-major
+product bit for bit.  So each problem's eigenvalues, off-diagonal, eigenvectors, iteration
+count and convergence flag are compared bit for bit with the oracle's qr_algorithm (layer
+-major apply) on that problem; full-size samples are checked against LAPACK and by
+residual / orthogonality.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(sep, ns, seed):
+    ns = np.asarray(ns, dtype=np.int64)
+    d, e, off_d, off_q = sep.inputs.batched_problems(ns, seed=seed)
+    return ns, d, e, off_d, off_q
+
+
+def _solve(sep, handle, ns, d, e, off_d, off_q, max_layers=0, tune=False):
+    import torch
+    tot_q = int(off_q[-1] + ns[-1] ** 2)
+    dvd = torch.from_numpy(d.copy()).cuda()
+    evd = torch.from_numpy(e.copy()).cuda()
+    Q = torch.zeros(tot_q, dtype=torch.float64, device="cuda")
+    for p, n in enumerate(ns):
+        Q[off_q[p]:off_q[p] + n * n].view(n, n).fill_diagonal_(1.0)
+    if tune:
+        sep.tune_batched(ns, off_d, off_q, dvd, evd, handle=handle)
+    conv, iters = sep.qr_batched_(ns, off_d, off_q, dvd, evd, Q, max_layers, handle=handle)
+    return conv, iters, dvd.cpu().numpy(), evd.cpu().numpy(), Q.cpu().numpy()
+
+
+def _check_bitwise(ns, d, e, off_d, off_q, ml, conv, iters, gd, ge, gQ, which=None):
+    for p in (range(len(ns)) if which is None else which):
+        n, o = int(ns[p]), off_d[p]
+        dv, ev = d[o:o + n].copy(), e[o:o + n - 1].copy()
+        Qo = np.asfortranarray(np.eye(n))
+        oconv, oit, _, _ = oracle.qr_algorithm(dv, ev, Qo, ml)
+        assert bool(conv[p]) == oconv and iters[p] == oit, (p, n)
+        assert np.array_equal(gd[o:o + n], dv), (p, n)
+        assert np.array_equal(ge[o:o + n - 1], ev), (p, n)
+        Qg = gQ[off_q[p]:off_q[p] + n * n].reshape(n, n).T   # column-major
+        assert np.array_equal(Qg, Qo), (p, n)
+
+
+@pytest.mark.parametrize("max_layers,fused_depth,rows_per_lane",
+                         [(0, 0, 0), (16, 8, 1), (20, 8, 2), (64, 8, 2), (16, 4, 1), (7, 4, 1)])
+def test_rounds_bitwise_equal_to_oracle(sep, handle, max_layers, fused_depth, rows_per_lane):
+    r = np.random.default_rng(21)
+    ns = (2 * r.integers(1, 70, size=40)).astype(np.int64)
+    ns[:4] = [2, 4, 130, 200]
+    print(f"[batched] max_layers={max_layers} KF={fused_depth} R={rows_per_lane} B={len(ns)} "
+          f"nmax={ns.max()}", flush=True)
+    ns, d, e, off_d, off_q = _batch(sep, ns, seed=13)
+    handle.set_batched_config(fused_depth, rows_per_lane)
+    try:
+        conv, iters, gd, ge, gQ = _solve(sep, handle, ns, d, e, off_d, off_q, max_layers)
+    finally:
+        handle.set_batched_config(0, 0)
+    ml = max_layers if max_layers > 0 else 32
+    _check_bitwise(ns, d, e, off_d, off_q, ml, conv, iters, gd, ge, gQ)
+
+
+def test_rounds_full_size_sample_with_tuner(sep, handle):
+    """Batched-solve workload shapes (even orders in [32, 512], d_i = i + N(0, 0.1),
+    e ~ U[0,1)): a 512-problem sample; the autotuner runs first.  LAPACK eigenvalues,
+    residual and orthogonality for every 8th problem; then a solve with max_layers = 32
+    under the tuned apply configuration, bitwise against the oracle for 12 problems."""
+    ns = sep.inputs.batched_orders(4096)[:512]
+    print(f"[batched] tuner + B={len(ns)}", flush=True)
+    ns, d, e, off_d, off_q = _batch(sep, ns, seed=5)
+    conv, iters, gd, ge, gQ = _solve(sep, handle, ns, d, e, off_d, off_q, 0, tune=True)
+    assert conv.all()
+    for p in range(0, len(ns), 8):
+        n, o = int(ns[p]), off_d[p]
+        lam = gd[o:o + n]
+        ref = sl.eigvalsh_tridiagonal(d[o:o + n], e[o:o + n - 1])
+        assert np.abs(np.sort(lam) - ref).max() <= 1e-12 * np.abs(ref).max()
+        Qg = gQ[off_q[p]:off_q[p] + n * n].reshape(n, n).T
+        A = np.diag(d[o:o + n]) + np.diag(e[o:o + n - 1], 1) + np.diag(e[o:o + n - 1], -1)
+        assert np.linalg.norm(A @ Qg - Qg * lam) <= 1e-13 * np.linalg.norm(A)
+        assert np.linalg.norm(Qg.T @ Qg - np.eye(n)) <= 1e-13 * n
+    # the tuned configuration changes only speed: an explicit block length gives the
+    # oracle's bits whatever fused depth / rows per lane the tuner chose
+    conv, iters, gd, ge, gQ = _solve(sep, handle, ns, d, e, off_d, off_q, 32)
+    _check_bitwise(ns, d, e, off_d, off_q, 32, conv, iters, gd, ge, gQ, which=range(0, 512, 43))
+
+
+def test_rounds_repeat_is_deterministic(sep, handle):
+    """Two solves of the same batch give identical bits."""
+    ns = np.array([12, 64, 100, 8, 256, 30], dtype=np.int64)
+    ns, d, e, off_d, off_q = _batch(sep, ns, seed=3)
+    a = _solve(sep, handle, ns, d, e, off_d, off_q, 32)
+    b = _solve(sep, handle, ns, d, e, off_d, off_q, 32)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_rounds_errors(sep, handle):
+    import torch
+    z = torch.zeros(64, dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):
+        handle.set_batched_config(4, 2)
+    with pytest.raises(AssertionError):
+        sep.qr_batched_(np.array([4, 5]), np.array([0, 4]), np.array([0, 16]), z, z, z,
+                        handle=handle)
+    with pytest.raises(ValueError):
+        sep.qr_batched_(np.array([4, 0]), np.array([0, 4]), np.array([0, 16]), z, z, z,
+                        handle=handle)
+    with pytest.raises(ValueError):
+        sep.qr_batched_(np.array([4]), np.array([0]), np.array([0]), z, z, z, max_layers=300,
+                        handle=handle)
+    conv, iters = sep.qr_batched_(np.array([], dtype=np.int64), np.array([], dtype=np.int64),
+                                  np.array([], dtype=np.int64), z, z, z, handle=handle)
+    assert conv.shape == (0,) and iters.shape == (0,)
