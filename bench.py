@@ -15,7 +15,7 @@ Also reported in the same line:
 * extra: the other apply shapes (n = 2000 Float64/Float32, n = 16384 Float32), the plain
   solve of the README example (n = 2000) beside the oracle's CPU solve, the
   differentiated-solve workload (n = 4096: forward solve, backward, and their ratio), and
-  the batched solver's status.
+  the batched workload (4096 problems: solves/s against the per-problem GPU loop).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
                       [--calibrate-copy-gib G]
@@ -97,7 +97,8 @@ def time_apply(sep, h, torch, n, k, dtype, steps=5):
 
 
 def extras(sep, h, torch, np, cpu=True):
-    """Secondary figures: other apply shapes, config-1 solve, the differentiated solve."""
+    """Secondary figures: other apply shapes, config-1 solve, the differentiated solve, the
+    batched solve."""
     sys.path.insert(0, ROOT)
     ex = {}
     for n, dt, name in ((2000, torch.float64, "apply_n2000_f64"),
@@ -160,8 +161,53 @@ def extras(sep, h, torch, np, cpu=True):
     log(f"[bench] AD n=4096: forward {fwd:.3f} s, backward {bwd * 1e3:.2f} ms "
         f"({ex['ad_n4096_backward_tflops']} TFLOP/s on 4 n^3), ratio {bwd / fwd:.5f}")
 
-    ex["batched"] = {"status": "disabled", "reason": "the persistent batched kernel is "
-                     "switched off in this build (a GPU run did not complete; cause not found)"}
+    # config 4: 4096 uneven problems, even n in [32, 512], d_i = i + N(0, 0.1), e ~ U[0,1)
+    B = 4096
+    ns = sep.inputs.batched_orders(B)
+    d, e, off_d, off_q = sep.inputs.batched_problems(ns, seed=5)
+    tot_q = int(off_q[-1] + ns[-1] ** 2)
+
+    def fresh():
+        dvb = torch.from_numpy(d.copy()).cuda()
+        evb = torch.from_numpy(e.copy()).cuda()
+        Qb = torch.zeros(tot_q, dtype=torch.float64, device="cuda")
+        for p, nn in enumerate(ns):
+            Qb[off_q[p]:off_q[p] + nn * nn].view(nn, nn).fill_diagonal_(1.0)
+        torch.cuda.synchronize()
+        return dvb, evb, Qb
+
+    dvb, evb, Qb = fresh()
+    t = time.perf_counter()
+    choice = sep.tune_batched(ns, off_d, off_q, dvb, evb, handle=h)
+    tune_s = time.perf_counter() - t
+    best = float("inf")
+    for _ in range(2):
+        dvb, evb, Qb = fresh()
+        t = time.perf_counter()
+        conv, iters = sep.qr_batched_(ns, off_d, off_q, dvb, evb, Qb, handle=h)
+        best = min(best, time.perf_counter() - t)
+    # the plain per-problem GPU loop (same library, one qr_algorithm! per problem) on every
+    # 64th problem, extrapolated to the batch by problem count
+    sample = list(range(0, B, 64))
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for p in sample:
+        nn, o = int(ns[p]), int(off_d[p])
+        Hs = sep.SymTridiagonal(torch.from_numpy(d[o:o + nn].copy()).cuda(),
+                                torch.from_numpy(e[o:o + nn - 1].copy()).cuda())
+        Qs = torch.eye(nn, dtype=torch.float64, device="cuda").t()
+        sep.qr_algorithm_(Hs, Qs, 32, handle=h)
+    torch.cuda.synchronize()
+    loop_rate = len(sample) / (time.perf_counter() - t)
+    ex["batched_solves_per_s"] = round(B / best, 1)
+    ex["batched_s"] = round(best, 4)
+    ex["batched_all_converged"] = bool(conv.all())
+    ex["batched_tuner_choice"] = int(choice)
+    ex["batched_tune_s"] = round(tune_s, 3)
+    ex["per_problem_loop_solves_per_s"] = round(loop_rate, 2)
+    ex["batched_over_loop"] = round(B / best / loop_rate, 1)
+    log(f"[bench] batched B={B}: {best * 1e3:.1f} ms = {B / best:.0f} solves/s; per-problem "
+        f"loop {loop_rate:.1f} solves/s; ratio {B / best / loop_rate:.0f}x")
     return ex
 
 

@@ -57,6 +57,9 @@ def _declare(lib: C.CDLL, pfx: str) -> None:
         f = getattr(lib, pfx + "orc_bulk_rmul_seq" + suf)
         f.argtypes = [_I64, _I64, _P, _I64, _P, _I64]
         f.restype = None
+        f = getattr(lib, pfx + "orc_bulk_rmul_seq_fma" + suf)
+        f.argtypes = [_I64, _I64, _P, _I64, _P, _I64]
+        f.restype = None
         f = getattr(lib, pfx + "orc_apply_log" + suf)
         f.argtypes = [_I64, _P, _I64, _P, _P, _P, _I64]
         f.restype = None
@@ -127,6 +130,14 @@ def bulk_rmul_seq(A: np.ndarray, grid: np.ndarray) -> None:
     _fn("orc_bulk_rmul_seq", A.dtype)(m, n, _ptr(A), m, _ptr(grid), k)
 
 
+def bulk_rmul_seq_fma(A: np.ndarray, grid: np.ndarray) -> None:
+    """Layer-major product with a1' = fma(a1, c, a2 s), a2' = fma(a1, -s, a2 c)."""
+    assert A.flags.f_contiguous and grid.flags.c_contiguous and A.dtype == grid.dtype
+    m, n = A.shape
+    k = grid.shape[0] if grid.size else 0
+    _fn("orc_bulk_rmul_seq_fma", A.dtype)(m, n, _ptr(A), m, _ptr(grid), k)
+
+
 def apply_log(A: np.ndarray, idx, c, s) -> None:
     idx = np.ascontiguousarray(idx, dtype=np.int64)
     c = np.ascontiguousarray(c, dtype=A.dtype)
@@ -153,7 +164,8 @@ def qr_algorithm(dv, ev, Q=None, max_layers: int = 128, tol=None, maxiter=None, 
     """qr_algorithm!(H, Q, max_layers, tol, maxiter); mutates dv, ev, Q.
 
     Returns (converged, iter, n_rotations, n_blocks).  apply_mode 0 = layer-major grid,
-    1 = wave order, 2 = chronological rotation log.  fast=True runs the FMA-contracted build
+    1 = wave order, 2 = chronological rotation log, 3 = layer-major with the rotation as
+    two fused multiply-adds.  fast=True runs the FMA-contracted build
     (the timed CPU baseline; parity checks use the default build)."""
     dt = dv.dtype
     n = dv.shape[0]

@@ -12,6 +12,7 @@
  *   orc_qr_algorithm_*      src/bulge_chasing.jl:146-288
  *   orc_chase_block_*       src/bulge_chasing.jl:166-284 (one outer iteration, no apply)
  *   orc_bulk_rmul_seq_*     layer-major Rotation2 rmul!, src/simple_givens.jl:53-64
+ *   orc_bulk_rmul_seq_fma_* the same with the rotation as two FMAs (src/bulk_givens.jl:44-57)
  *   orc_bulk_wave_*         src/wave.jl:62-162 with src/bulk_givens.jl:4-279
  *   orc_bulk_wave_parallel_* src/wave.jl:36-60
  *
@@ -53,6 +54,7 @@
 #define orc_single_shift PFX(orc_single_shift)
 #define orc_wilkinson_shift PFX(orc_wilkinson_shift)
 #define orc_bulk_rmul_seq PFX(orc_bulk_rmul_seq)
+#define orc_bulk_rmul_seq_fma PFX(orc_bulk_rmul_seq_fma)
 #define orc_apply_log PFX(orc_apply_log)
 #define orc_bulk_wave PFX(orc_bulk_wave)
 #define orc_bulk_wave_parallel PFX(orc_bulk_wave_parallel)

@@ -219,6 +219,32 @@ void FN(orc_bulk_rmul_seq)(int64_t m, int64_t n, T *A, int64_t lda, const T *cs,
         }
 }
 
+/* The same layer-major product with the rotation evaluated as two fused multiply-adds,
+ * a1' = fma(a1, c, a2 s), a2' = fma(a1, -s, a2 c): one reading of the reference's
+ * @fastmath / SIMD contraction of `s * col3 + c * col2` (src/bulk_givens.jl:44-57), and
+ * the form the batched GPU solver computes. */
+static inline void FN(rmul_one_fma)(int64_t m, T *A, int64_t lda, int64_t i, T c, T s)
+{
+    T *a1 = A + (i - 1) * lda;
+    T *a2 = A + i * lda;
+    for (int64_t j = 0; j < m; j++) {
+        T x = a1[j], y = a2[j];
+        a1[j] = FMA(x, c, y * s);
+        a2[j] = FMA(x, -s, y * c);
+    }
+}
+
+void FN(orc_bulk_rmul_seq_fma)(int64_t m, int64_t n, T *A, int64_t lda, const T *cs, int64_t k)
+{
+    for (int64_t L = 1; L <= k; L++)
+        for (int64_t i = 1; i <= n - 1; i++) {
+            T c = cs[2 * ((i - 1) + (n - 1) * (L - 1))];
+            T s = cs[2 * ((i - 1) + (n - 1) * (L - 1)) + 1];
+            if (s == 0) continue;
+            FN(rmul_one_fma)(m, A, lda, i, c, s);
+        }
+}
+
 /* Apply a chronological rotation log one rotation at a time. */
 void FN(orc_apply_log)(int64_t m, T *A, int64_t lda, const int64_t *idx, const T *c,
                        const T *s, int64_t len)
@@ -406,7 +432,8 @@ int FN(orc_bulk_wave_parallel)(int64_t m, int64_t N, T *A, int64_t lda, const T 
  * success, -1 for odd n (the @assert at :155); *converged / *iter as the
  * reference's return tuple; *nrot counts the rotations the chase created.
  * apply_mode: 0 = layer-major sequential grid apply, 1 = wave order (requires the
- * wave preconditions, k even and k <= n-2), 2 = chronological log (no grid). */
+ * wave preconditions, k even and k <= n-2), 2 = chronological log (no grid),
+ * 3 = layer-major with the fused-multiply-add rotation. */
 int FN(orc_qr_algorithm)(int64_t n, T *dv, T *ev, T *Q, int64_t ldq, int64_t max_layers, T tol,
                          int64_t maxiter, int32_t *converged, int64_t *iter_out, int64_t *nrot,
                          int apply_mode, int64_t *nblocks)
@@ -473,6 +500,8 @@ int FN(orc_qr_algorithm)(int64_t n, T *dv, T *ev, T *Q, int64_t ldq, int64_t max
         if (Q) {
             if (apply_mode == 0) {
                 FN(orc_bulk_rmul_seq)(n, n, Q, ldq, givens, max_layers);
+            } else if (apply_mode == 3) {
+                FN(orc_bulk_rmul_seq_fma)(n, n, Q, ldq, givens, max_layers);
             } else if (apply_mode == 1) {
                 if (FN(orc_bulk_wave)(n, n, Q, ldq, givens, max_layers) != 0) {
                     ret = -3;

@@ -2,24 +2,31 @@
 // counterpart in the reference, which solves one matrix per call).
 //
 // The outer loop of qr_algorithm! (src/bulge_chasing.jl:166-285) runs for all problems at
-// once.  One round is two launches over the whole batch:
+// once, in rounds.  Round r is two launches over the whole batch:
 //   k_chase_all:  one wavefront per problem runs one block of sweeps with the single
-//                 solve's device chase (chase.hpp, src/bulge_chasing.jl:166-276), in the
+//                 solve's device chase (chase.hpp, src/bulge_chasing.jl:166-276), into the
 //                 reference's (n-1) x max_layers grid layout; a problem that has finished
-//                 (no sweep in its last block, or maxiter exceeded) returns at once;
-//   k_apply_all:  the block's rotations are applied to each active problem's Q in wave
-//                 order t = i + 2L (src/wave.jl:62-162), KF layers per pass over Q, R rows
-//                 of Q per lane, the window of 2 KF columns per row in VGPRs, and the
-//                 (c, s) of 16 time steps staged in LDS at a time.  Rotations beyond a
-//                 column's layer count are read as the identity, so the grid never needs
-//                 an identity fill.  The rotation is Rotation2 rmul! (src/simple_givens.jl:
-//                 53-64) operation for operation, so Q equals the layer-major product of
-//                 the C restatement bit for bit.
-// The host replays rounds until a device counter says every problem is done (every
-// round either finishes a problem or advances its iteration count, so the number of
-// rounds is bounded by max maxiter + 2).  The autotuner picks max_layers, the fused depth
-// KF and the rows per lane R.
+//                 (no sweep in its last block, or maxiter exceeded) returns at once.  It
+//                 leaves the depth to apply (0 when nothing is to be applied) and the
+//                 column range the block touched, per problem;
+//   k_apply_all:  the block's rotations are applied to each problem's Q in wave order
+//                 t = i + 2L (src/wave.jl:62-162), KF layers per pass over Q, R rows of Q
+//                 per lane with the window of 2 KF columns in VGPRs, Q columns prefetched
+//                 8 steps ahead, and the (c, s) of 16 time steps staged in LDS (double
+//                 buffered, loaded one stage ahead).  Rotations beyond a column's layer
+//                 count read as the identity, so the grid never needs an identity fill.
+//                 The rotation is a1' = fma(a1, c, a2 s), a2' = fma(a1, -s, a2 c) (Rotation2
+//                 rmul!, src/simple_givens.jl:53-64, contracted as the reference's SIMD
+//                 kernels may be, src/bulk_givens.jl:44-57), so Q equals the C
+//                 restatement's layer-major product with the same rotation bit for bit.
+// The grid, layer counts and depths are double buffered by round parity, so the apply of
+// round r (apply stream) overlaps the chase of round r+1 (chase stream).  The host reads a
+// device count of finished problems after each chase; every round either finishes a
+// problem or advances its iteration count, so the number of rounds is at most
+// max(maxiter) + 2.  The autotuner picks max_layers, the fused depth KF and the rows per
+// lane R.
 #include <algorithm>
+#include <cstdlib>
 #include <numeric>
 #include <type_traits>
 
@@ -31,6 +38,7 @@ namespace {
 
 constexpr int kLanes = 64;
 constexpr int kStageSteps = 16;  // wave-order time steps staged per LDS fill
+constexpr int kQAhead = 8;       // Q columns prefetched ahead
 constexpr int kMaxBatchedN = 8192;
 
 // state word (beyond the chase's own) marking a finished problem
@@ -51,24 +59,46 @@ struct BWork {
     int row0;
 };
 
-// One block of sweeps for every unfinished problem (one wavefront each).
+// One block of sweeps for every unfinished problem (one wavefront each).  lt / grid /
+// depth are this round's buffers.
 __global__ void __launch_bounds__(kLanes)
     k_chase_all(const BProb* __restrict__ probs, double* dv, double* ev, int* lt, double* grid,
-                int64_t* st_all, int ML, int* done_count)
+                int4* info, int64_t* st_all, int ML, int nmax, int* done_count)
 {
     const int p = blockIdx.x;
     const int lane = threadIdx.x;
     int64_t* st = st_all + (int64_t)p * chase::ST_WORDS;
-    if (__builtin_amdgcn_readfirstlane((int)st[ST_DONE])) return;
+    if (__builtin_amdgcn_readfirstlane((int)st[ST_DONE])) {
+        if (lane == 0) info[p] = make_int4(0, 0, 0, 0);
+        return;
+    }
     const BProb pr = probs[p];
+    // The serial chase is the critical path, and a round lasts as long as its largest
+    // problems' blocks: chase waves issue ahead of the concurrent apply, larger problems
+    // ahead of smaller ones on a shared SIMD.
+    const int pri = __builtin_amdgcn_readfirstlane((4 * pr.n) / (nmax + 1));
+    if (pri >= 3)
+        __builtin_amdgcn_s_setprio(3);
+    else if (pri == 2)
+        __builtin_amdgcn_s_setprio(2);
+    else
+        __builtin_amdgcn_s_setprio(1);
     chase::Problem<double> P{dv + pr.off_d, ev + pr.off_d, lt + pr.off_lt, grid + 2 * pr.off_g,
                              pr.n, ML, (int64_t)(pr.n - 1), 0};
     const double tol = 2.220446049250313e-16;  // eps(Float64)
     chase::chase_block(P, tol, 100 * (int64_t)pr.n, st, lane);
     __threadfence_block();
     if (lane == 0) {
-        const bool done = st[chase::ST_EARLY] != 0 || st[chase::ST_DID] == 0;
-        if (done) {
+        // maxiter exceeded: the reference returns without applying this block
+        // (src/bulge_chasing.jl:180-182); no sweep: nothing to apply and done (:279)
+        const bool early = st[chase::ST_EARLY] != 0, did = st[chase::ST_DID] != 0;
+        // apply depth and the column range the block touched: rotation indices
+        // [ilo, ihi] (1-based) act on columns ilo - 1 .. ihi (0-based)
+        const int ilo = (int)st[chase::ST_ILO], ihi = (int)st[chase::ST_IHI];
+        info[p] = (did && !early)
+                      ? make_int4((int)st[chase::ST_MAXLAYER], ilo - 1, ihi - ilo + 2, 0)
+                      : make_int4(0, 0, 0, 0);
+        if (early || !did) {
             st[ST_DONE] = 1;
             atomicAdd(done_count, 1);
         }
@@ -87,101 +117,152 @@ struct Unroll {
     }
 };
 
-#pragma clang fp contract(off)
+// Buffer resource over `bytes` bytes at p: loads beyond return 0 and stores beyond are
+// dropped by the hardware range check (raw buffer, word 3 as for every gfx9 part).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(void* p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ double buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, double v)
+{
+    using u2 = unsigned __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, off, 0, 0);
+}
 
-// Apply the block's rotations to Q for rows [row0, row0 + 64 R) of one problem.
+// Apply this round's block to Q for rows [row0, row0 + 64 R) of one problem, over the
+// columns c0 .. c0 + nc - 1 the block touched (the others are unchanged by it).  Q is
+// addressed through a buffer resource spanning exactly those columns, so the wave-order
+// loads of columns past the range read 0 and the stores of columns outside it (and of
+// lanes past the last row) are dropped without a branch.
 template <int KF, int R>
 __global__ void __launch_bounds__(kLanes)
     k_apply_all(const BWork* __restrict__ work, const BProb* __restrict__ probs,
                 const int* __restrict__ lt, const double* __restrict__ grid,
-                const int64_t* __restrict__ st_all, double* __restrict__ Q)
+                const int4* __restrict__ info, double* __restrict__ Q)
 {
     constexpr int W = 2 * KF;
     constexpr int SB = kStageSteps;
-    static_assert(SB % W == 0, "staged steps must be a multiple of the window");
-    __shared__ double2 stg[SB][KF];
+    constexpr int PA = kQAhead;
+    constexpr int NE = SB * KF / kLanes;  // staged entries per lane
+    static_assert(SB % W == 0 && SB % PA == 0 && NE * kLanes == SB * KF, "stage geometry");
+    __shared__ double2 stg[2][SB][KF];
 
     const BWork wk = work[blockIdx.x];
-    const int64_t* st = st_all + (int64_t)wk.prob * chase::ST_WORDS;
-    // nothing to apply: finished (no sweep, or maxiter exceeded: the reference returns
-    // without applying that block, src/bulge_chasing.jl:180-182)
-    if (__builtin_amdgcn_readfirstlane((int)st[ST_DONE])) return;
-    const int depth = __builtin_amdgcn_readfirstlane((int)st[chase::ST_MAXLAYER]);
+    const int4 inf = info[wk.prob];
+    const int depth = __builtin_amdgcn_readfirstlane(inf.x);
+    if (depth == 0) return;
+    const int c0 = __builtin_amdgcn_readfirstlane(inf.y);
+    const int nc = __builtin_amdgcn_readfirstlane(inf.z);
     const BProb pr = probs[wk.prob];
     const int n = pr.n;
     const int lane = threadIdx.x;
     const int* __restrict__ ltp = lt + pr.off_lt;
     const double* __restrict__ g = grid + 2 * pr.off_g;
-    double* __restrict__ Qp = Q + pr.off_q;
-    int rowi[R];
-    bool act[R];
+    const __amdgpu_buffer_rsrc_t qr =
+        make_rsrc(Q + pr.off_q + (int64_t)c0 * n, (uint32_t)n * (uint32_t)nc * 8u);
+    const uint32_t colb = (uint32_t)n * 8u;  // bytes per column
+    // byte offset of this lane's row; lanes past the last row get an offset beyond any
+    // range (2^31 > 8 n nc + every column offset used), so they load 0 and store nothing
+    uint32_t rowb[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int row = wk.row0 + r * kLanes + lane;
-        act[r] = row < n;
-        rowi[r] = min(row, n - 1);
+        rowb[r] = row < n ? 8u * (uint32_t)row : 0x80000000u;
     }
-    const int Tn = n + 2 * KF - 2;  // time steps of one pass, the last one only stores
+    // staged entry e of this lane: step s = q / KF, layer l = q % KF with q = lane + 64 e
+    double2 sv[NE];
+    int slt[NE];
+    bool sin_[NE];
+    auto stage_load = [&](int t0, int L0) {
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const int q = lane + kLanes * e;
+            const int s = q / KF, l = q % KF;
+            const int il = t0 + s - 2 * l, L = L0 + l;
+            sin_[e] = il >= 0 && il <= nc - 2 && L < depth;
+            const int ic = c0 + min(max(il, 0), nc - 2), Lc = min(L, depth - 1);
+            slt[e] = ltp[ic];
+            sv[e] = *reinterpret_cast<const double2*>(g + 2 * (ic + (int64_t)(n - 1) * Lc));
+        }
+    };
+    auto stage_store = [&](int buf, int L0) {
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const int q = lane + kLanes * e;
+            const int s = q / KF, l = q % KF;
+            stg[buf][s][l] = (sin_[e] && L0 + l < slt[e]) ? sv[e] : make_double2(1.0, 0.0);
+        }
+    };
+
+    // time steps of one pass (the last ones only store), rounded up to whole stages: steps
+    // past the end meet identity rotations and columns beyond the range
+    const int nblk = (nc + 2 * KF - 2 + SB - 1) / SB;
     for (int L0 = 0; L0 < depth; L0 += KF) {
         double win[R][W];
+        double qn[R][PA];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
 #pragma unroll
             for (int w = 0; w < W; ++w) win[r][w] = 0.0;
-            win[r][0] = act[r] ? Qp[rowi[r]] : 0.0;
+            win[r][0] = buf_load(qr, rowb[r]);
+#pragma unroll
+            for (int k = 0; k < PA; ++k) qn[r][k] = buf_load(qr, rowb[r] + (uint32_t)(k + 1) * colb);
         }
-        for (int t0 = 0; t0 < Tn; t0 += SB) {
-            __syncthreads();  // the previous fill has been read
-            for (int q = lane; q < SB * KF; q += kLanes) {
-                const int s = q / KF, l = q % KF;
-                const int i = t0 + s - 2 * l, L = L0 + l;
-                double2 v = make_double2(1.0, 0.0);
-                if (i >= 0 && i <= n - 2 && L < depth && L < ltp[i])
-                    v = *reinterpret_cast<const double2*>(g + 2 * (i + (int64_t)(n - 1) * L));
-                stg[s][l] = v;
-            }
+        __syncthreads();  // the previous pass has read both stage buffers
+        stage_load(0, L0);
+        for (int b = 0; b < nblk; ++b) {
+            const int buf = b & 1;
+            const int t0 = b * SB;
+            stage_store(buf, L0);
             __syncthreads();
+            if (b + 1 < nblk) stage_load(t0 + SB, L0);  // in flight during this stage
+            // byte offsets of the columns loaded / stored at step 0 of this stage; a
+            // negative column wraps to an offset beyond the range
+            const uint32_t ld0 = (uint32_t)(t0 + 1 + PA) * colb;
+            const uint32_t st0 = (uint32_t)(t0 - 2 * KF + 2) * colb;
             Unroll<SB, 0>::run([&](auto sc) {
                 constexpr int s = decltype(sc)::value;
-                const int t = t0 + s;
-                if (t < Tn) {
-                    const int jn = t + 1;  // column entering the window
+                constexpr int k = s % PA;
 #pragma unroll
-                    for (int r = 0; r < R; ++r)
-                        win[r][(s + 1) % W] =
-                            (act[r] && jn < n) ? Qp[rowi[r] + (int64_t)jn * n] : 0.0;
-                    Unroll<KF, 0>::run([&](auto lc) {
-                        constexpr int l = decltype(lc)::value;
-                        constexpr int ia = ((s - 2 * l) % W + W) % W;
-                        constexpr int ib = (ia + 1) % W;
-                        const double2 cs = stg[s][l];
-#pragma unroll
-                        for (int r = 0; r < R; ++r) {
-                            const double x = win[r][ia], y = win[r][ib];
-                            win[r][ia] = x * cs.x + y * cs.y;
-                            win[r][ib] = x * -cs.y + y * cs.x;
-                        }
-                    });
-                    const int jo = t - 2 * KF + 2;  // column leaving the window
-                    if (jo >= 0 && jo < n) {
-#pragma unroll
-                        for (int r = 0; r < R; ++r)
-                            if (act[r]) Qp[rowi[r] + (int64_t)jo * n] = win[r][(s + 2) % W];
-                    }
+                for (int r = 0; r < R; ++r) {
+                    win[r][(s + 1) % W] = qn[r][k];  // column t + 1 enters the window
+                    qn[r][k] = buf_load(qr, rowb[r] + ld0 + (uint32_t)s * colb);
                 }
+                Unroll<KF, 0>::run([&](auto lc) {
+                    constexpr int l = decltype(lc)::value;
+                    constexpr int ia = ((s - 2 * l) % W + W) % W;
+                    constexpr int ib = (ia + 1) % W;
+                    const double2 cs = stg[buf][s][l];
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const double x = win[r][ia], y = win[r][ib];
+                        win[r][ia] = __builtin_fma(x, cs.x, y * cs.y);
+                        win[r][ib] = __builtin_fma(x, -cs.y, y * cs.x);
+                    }
+                });
+#pragma unroll
+                for (int r = 0; r < R; ++r)  // column t - 2 KF + 2 leaves the window
+                    buf_store(qr, rowb[r] + st0 + (uint32_t)s * colb, win[r][(s + 2) % W]);
             });
         }
     }
 }
 
-#pragma clang fp contract(on)
-
+// max_layers, fused depth, rows per lane, and chase waves per SIMD (the chase kernel
+// reserves LDS so that at most CW of its waves share a SIMD: a chase wave's serial
+// recurrence slows down in proportion to the chase waves beside it)
 struct Config {
-    int ML, KF, R;
+    int ML, KF, R, CW;
 };
 constexpr Config kConfigs[] = {
-    {32, 8, 1}, {32, 8, 2}, {64, 8, 1}, {64, 8, 2}, {16, 8, 1}, {32, 4, 1}, {16, 4, 1},
+    {32, 8, 2, 4}, {32, 8, 2, 2}, {32, 8, 2, 1}, {64, 8, 2, 2}, {64, 8, 2, 1},
+    {32, 8, 1, 2}, {16, 8, 1, 2}, {64, 8, 1, 1}, {32, 4, 1, 2},
 };
+constexpr int kLdsPerCU = 160 * 1024;
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 
 struct Layout {
@@ -246,34 +327,55 @@ void build_layout(int64_t B, const int64_t* n, const int64_t* off_d, const int64
 
 template <int KF, int R>
 void launch_apply(int nwork, hipStream_t s, const BWork* w, const BProb* p, const int* lt,
-                  const double* g, const int64_t* st, double* Q)
+                  const double* g, const int4* depth, double* Q)
 {
-    hipLaunchKernelGGL((k_apply_all<KF, R>), dim3(nwork), dim3(kLanes), 0, s, w, p, lt, g, st, Q);
+    hipLaunchKernelGGL((k_apply_all<KF, R>), dim3(nwork), dim3(kLanes), 0, s, w, p, lt, g, depth,
+                       Q);
 }
 
-// The whole batched solve on the handle's stream; returns after completion.
+// Events and pinned counter of the round pipeline, created on first use.
+int ensure_batch_sync(sep_handle* h)
+{
+    if (!h->batch_pinned)
+        SEP_HIP(hipHostMalloc((void**)&h->batch_pinned, sizeof(int) * 4, hipHostMallocDefault));
+    for (int i = 0; i < 4; ++i)
+        if (!h->batch_ev[i]) SEP_HIP(hipEventCreateWithFlags(&h->batch_ev[i], hipEventDisableTiming));
+    return SEP_OK;
+}
+
+// The whole batched solve; returns after completion.  Round r uses buffer set r & 1:
+// chase(r) on the chase stream waits for apply(r - 2) (same buffers), apply(r) on the
+// apply stream waits for chase(r).
 int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d,
                 const int64_t* off_q, double* dv, double* ev, double* Q, const Config& cfg,
                 int32_t* converged, int64_t* iters)
 {
     Layout L;
     build_layout(B, n, off_d, off_q, cfg, L);
+    if (int rc = ensure_batch_sync(h)) return rc;
     const size_t probs_b = sizeof(BProb) * (size_t)B;
     const size_t work_b = sizeof(BWork) * L.work.size();
     const size_t st_b = sizeof(int64_t) * chase::ST_WORDS * (size_t)B;
+    const size_t depth_b = sizeof(int4) * (size_t)B;
     auto al = [](size_t x) { return (x + 255) / 256 * 256; };
-    if (int rc = h->batch_meta.reserve(al(probs_b) + al(work_b) + al(st_b) + 256)) return rc;
-    if (int rc = h->batch_scratch.reserve(al(sizeof(int) * (size_t)L.tot_lt) +
-                                          sizeof(double) * 2 * (size_t)L.tot_g + 256))
+    if (int rc = h->batch_meta.reserve(al(probs_b) + al(work_b) + al(st_b) + 2 * al(depth_b) +
+                                       256))
         return rc;
+    const size_t lt_set = al(sizeof(int) * (size_t)L.tot_lt);
+    const size_t g_set = al(sizeof(double) * 2 * (size_t)L.tot_g);
+    if (int rc = h->batch_scratch.reserve(2 * (lt_set + g_set))) return rc;
     char* m = static_cast<char*>(h->batch_meta.p);
     BProb* d_probs = reinterpret_cast<BProb*>(m);
     BWork* d_work = reinterpret_cast<BWork*>(m + al(probs_b));
     int64_t* d_st = reinterpret_cast<int64_t*>(m + al(probs_b) + al(work_b));
-    int* d_count = reinterpret_cast<int*>(m + al(probs_b) + al(work_b) + al(st_b));
+    int4* d_depth[2];
+    d_depth[0] = reinterpret_cast<int4*>(m + al(probs_b) + al(work_b) + al(st_b));
+    d_depth[1] = reinterpret_cast<int4*>(reinterpret_cast<char*>(d_depth[0]) + al(depth_b));
+    int* d_count = reinterpret_cast<int*>(reinterpret_cast<char*>(d_depth[1]) + al(depth_b));
     char* sc = static_cast<char*>(h->batch_scratch.p);
-    int* d_lt = reinterpret_cast<int*>(sc);
-    double* d_grid = reinterpret_cast<double*>(sc + al(sizeof(int) * (size_t)L.tot_lt));
+    int* d_lt[2] = {reinterpret_cast<int*>(sc), reinterpret_cast<int*>(sc + lt_set)};
+    double* d_grid[2] = {reinterpret_cast<double*>(sc + 2 * lt_set),
+                         reinterpret_cast<double*>(sc + 2 * lt_set + g_set)};
 
     // host staging lives in the handle; every copy below completes before it is reused
     h->batch_host_st.assign((size_t)chase::ST_WORDS * B, 0);
@@ -283,37 +385,64 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
         s[chase::ST_LBEA] = n[p];
         s[chase::ST_ITER] = 0;
     }
-    int done = 0;
-    SEP_HIP(hipMemcpyAsync(d_probs, L.probs.data(), probs_b, hipMemcpyHostToDevice, h->stream));
-    SEP_HIP(hipMemcpyAsync(d_work, L.work.data(), work_b, hipMemcpyHostToDevice, h->stream));
-    SEP_HIP(hipMemcpyAsync(d_st, h->batch_host_st.data(), st_b, hipMemcpyHostToDevice, h->stream));
-    SEP_HIP(hipMemcpyAsync(d_count, &done, sizeof(int), hipMemcpyHostToDevice, h->stream));
-    SEP_HIP(hipStreamSynchronize(h->stream));
+    hipStream_t cs = h->chase_stream, as = h->stream;
+    int* pin = h->batch_pinned;
+    pin[0] = 0;
+    SEP_HIP(hipMemcpyAsync(d_probs, L.probs.data(), probs_b, hipMemcpyHostToDevice, as));
+    SEP_HIP(hipMemcpyAsync(d_work, L.work.data(), work_b, hipMemcpyHostToDevice, as));
+    SEP_HIP(hipMemcpyAsync(d_st, h->batch_host_st.data(), st_b, hipMemcpyHostToDevice, as));
+    SEP_HIP(hipMemcpyAsync(d_count, pin, sizeof(int), hipMemcpyHostToDevice, as));
+    SEP_HIP(hipStreamSynchronize(as));  // also orders the caller's dv / ev / Q before the chase
 
+    hipEvent_t* chased = h->batch_ev;     // [2] by round parity
+    hipEvent_t* applied = h->batch_ev + 2;
     const int nwork = (int)L.work.size();
     const int64_t max_rounds = 100 * (int64_t)L.nmax + 3;
     int64_t round = 0;
+    int done = 0;
+    int rc = SEP_OK;
     for (; round < max_rounds && done < B; ++round) {
-        hipLaunchKernelGGL(k_chase_all, dim3((unsigned)B), dim3(kLanes), 0, h->stream, d_probs,
-                           dv, ev, d_lt, d_grid, d_st, cfg.ML, d_count);
-        SEP_CHECK_LAUNCH();
+        const int b = (int)(round & 1);
+        if (round >= 2 && hipStreamWaitEvent(cs, applied[b], 0) != hipSuccess) {
+            rc = SEP_ERR_HIP;
+            break;
+        }
+        const unsigned reserve = cfg.CW >= 4 ? 0u : (unsigned)(kLdsPerCU / (4 * cfg.CW));
+        hipLaunchKernelGGL(k_chase_all, dim3((unsigned)B), dim3(kLanes), reserve, cs, d_probs, dv, ev,
+                           d_lt[b], d_grid[b], d_depth[b], d_st, cfg.ML, L.nmax, d_count);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(pin, d_count, sizeof(int), hipMemcpyDeviceToHost, cs) != hipSuccess ||
+            hipEventRecord(chased[b], cs) != hipSuccess ||
+            hipStreamWaitEvent(as, chased[b], 0) != hipSuccess) {
+            rc = SEP_ERR_HIP;
+            break;
+        }
         if (cfg.KF == 8 && cfg.R == 2)
-            launch_apply<8, 2>(nwork, h->stream, d_work, d_probs, d_lt, d_grid, d_st, Q);
+            launch_apply<8, 2>(nwork, as, d_work, d_probs, d_lt[b], d_grid[b], d_depth[b], Q);
         else if (cfg.KF == 8)
-            launch_apply<8, 1>(nwork, h->stream, d_work, d_probs, d_lt, d_grid, d_st, Q);
+            launch_apply<8, 1>(nwork, as, d_work, d_probs, d_lt[b], d_grid[b], d_depth[b], Q);
         else
-            launch_apply<4, 1>(nwork, h->stream, d_work, d_probs, d_lt, d_grid, d_st, Q);
-        SEP_CHECK_LAUNCH();
-        SEP_HIP(hipMemcpyAsync(&done, d_count, sizeof(int), hipMemcpyDeviceToHost, h->stream));
-        SEP_HIP(hipStreamSynchronize(h->stream));
+            launch_apply<4, 1>(nwork, as, d_work, d_probs, d_lt[b], d_grid[b], d_depth[b], Q);
+        if (hipGetLastError() != hipSuccess || hipEventRecord(applied[b], as) != hipSuccess ||
+            hipEventSynchronize(chased[b]) != hipSuccess) {
+            rc = SEP_ERR_HIP;
+            break;
+        }
+        done = pin[0];
+    }
+    if (hipStreamSynchronize(as) != hipSuccess || hipStreamSynchronize(cs) != hipSuccess)
+        rc = SEP_ERR_HIP;
+    if (rc) {
+        set_error("batched: a HIP call failed in round %lld", (long long)round);
+        return rc;
     }
     if (done < B) {
         set_error("batched: %d of %lld problems unfinished after %lld rounds", (int)(B - done),
                   (long long)B, (long long)round);
         return SEP_ERR_HIP;
     }
-    SEP_HIP(hipMemcpyAsync(h->batch_host_st.data(), d_st, st_b, hipMemcpyDeviceToHost, h->stream));
-    SEP_HIP(hipStreamSynchronize(h->stream));
+    SEP_HIP(hipMemcpyAsync(h->batch_host_st.data(), d_st, st_b, hipMemcpyDeviceToHost, as));
+    SEP_HIP(hipStreamSynchronize(as));
     std::vector<int32_t> conv((size_t)B);
     std::vector<int64_t> its((size_t)B);
     int all = 1;
@@ -325,11 +454,10 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
     }
     if (converged)
         SEP_HIP(hipMemcpyAsync(converged, conv.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice,
-                               h->stream));
+                               as));
     if (iters)
-        SEP_HIP(hipMemcpyAsync(iters, its.data(), sizeof(int64_t) * B, hipMemcpyHostToDevice,
-                               h->stream));
-    SEP_HIP(hipStreamSynchronize(h->stream));
+        SEP_HIP(hipMemcpyAsync(iters, its.data(), sizeof(int64_t) * B, hipMemcpyHostToDevice, as));
+    SEP_HIP(hipStreamSynchronize(as));
     return all ? SEP_OK : SEP_NOT_CONVERGED;
 }
 
@@ -462,6 +590,9 @@ int tune_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const int64_t* 
         if ((he = hipEventSynchronize(e1)) != hipSuccess) break;
         float ms = 0;
         if ((he = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+        if (std::getenv("SEP_TUNE_VERBOSE"))
+            std::fprintf(stderr, "[sep tune] config %d (ML %d KF %d R %d CW %d): %.2f ms\n", c,
+                         kConfigs[c].ML, kConfigs[c].KF, kConfigs[c].R, kConfigs[c].CW, ms);
         if (ms < best) {
             best = ms;
             best_i = c;

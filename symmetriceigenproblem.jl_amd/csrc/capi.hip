@@ -282,6 +282,9 @@ int sep_destroy(sep_handle* h)
     h->bwd_M.release();
     h->bwd_part.release();
     for (auto e : h->tev) (void)hipEventDestroy(e);
+    for (auto e : h->batch_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (h->batch_pinned) (void)hipHostFree(h->batch_pinned);
     if (h->ev_a) (void)hipEventDestroy(h->ev_a);
     if (h->ev_b) (void)hipEventDestroy(h->ev_b);
     if (h->chase_stream) (void)hipStreamDestroy(h->chase_stream);

@@ -27,6 +27,9 @@ __global__ void __launch_bounds__(64) k_chase_block(T* dv, T* ev, int* lt, T* gr
                                                     int max_layers, T tol, int64_t maxiter,
                                                     int64_t* st)
 {
+    // the chase is the solve's critical path: its wave issues ahead of the apply of the
+    // previous block running beside it
+    __builtin_amdgcn_s_setprio(3);
     chase::Problem<T> P{dv, ev, lt, grid, n, max_layers, (int64_t)(n - 1), 0};
     chase::chase_block(P, tol, maxiter, st, (int)threadIdx.x);
 }

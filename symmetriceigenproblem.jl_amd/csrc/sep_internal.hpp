@@ -91,6 +91,8 @@ struct sep_handle {
     void* bwd_meta_dev = nullptr;
     // batched solver: host copy of the per-problem state words, autotuner choice
     std::vector<int64_t> batch_host_st;
+    int* batch_pinned = nullptr;             // pinned finished-problem count
+    hipEvent_t batch_ev[4] = {nullptr, nullptr, nullptr, nullptr};  // chased[2], applied[2]
     std::vector<int64_t> tune_key;
     int tune_choice = -1;
     int force_kf = 0, force_r = 0;  // sep_set_batched_config (0: autotuned / default)

@@ -1,12 +1,13 @@
 """GPU parity of the round-based batched solver (no counterpart in the reference).
 
 Every problem of a batch runs qr_algorithm! semantics (src/bulge_chasing.jl:146-288) with
-the single solve's device chase, and its block of rotations is applied with Rotation2
-rmul! arithmetic (src/simple_givens.jl:53-64) in wave order, which equals the layer-major
-product bit for bit.  So each problem's eigenvalues, off-diagonal, eigenvectors, iteration
-count and convergence flag are compared bit for bit with the oracle's qr_algorithm (layer
--major apply) on that problem; full-size samples are checked against LAPACK and by
-residual / orthogonality.
+the single solve's device chase, and its block of rotations is applied in wave order with
+the rotation as two fused multiply-adds (Rotation2 rmul!, src/simple_givens.jl:53-64, as
+the reference's SIMD kernels contract it, src/bulk_givens.jl:44-57); wave order equals the
+layer-major product bit for bit.  So each problem's eigenvalues, off-diagonal,
+eigenvectors, iteration count and convergence flag are compared bit for bit with the
+oracle's qr_algorithm with the same rotation (apply_mode=3); full-size samples are checked
+against LAPACK and by residual / orthogonality.
 """
 import numpy as np
 import pytest
@@ -42,7 +43,7 @@ def _check_bitwise(ns, d, e, off_d, off_q, ml, conv, iters, gd, ge, gQ, which=No
         n, o = int(ns[p]), off_d[p]
         dv, ev = d[o:o + n].copy(), e[o:o + n - 1].copy()
         Qo = np.asfortranarray(np.eye(n))
-        oconv, oit, _, _ = oracle.qr_algorithm(dv, ev, Qo, ml)
+        oconv, oit, _, _ = oracle.qr_algorithm(dv, ev, Qo, ml, apply_mode=3)
         assert bool(conv[p]) == oconv and iters[p] == oit, (p, n)
         assert np.array_equal(gd[o:o + n], dv), (p, n)
         assert np.array_equal(ge[o:o + n - 1], ev), (p, n)

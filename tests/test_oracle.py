@@ -128,6 +128,31 @@ def test_bulk_rmul_seq_equals_dense_product():
     assert np.allclose(out, A @ P, atol=1e-14, rtol=0)
 
 
+def test_bulk_rmul_seq_fma_equals_dense_product_and_plain_form():
+    """The fused-multiply-add rotation form: equals the dense product of rotation matrices
+    to rounding, differs from the plain form only at rounding level, and leaves identity
+    entries (s == 0) exactly untouched."""
+    r = np.random.default_rng(31)
+    m, n, k = 7, 9, 5
+    A = np.asfortranarray(r.standard_normal((m, n)))
+    th = r.random((k, n - 1)) * 2 * np.pi
+    g = np.ascontiguousarray(np.stack([np.cos(th), np.sin(th)], axis=-1))
+    g[2, 3] = (1.0, 0.0)
+    D = A.copy()
+    for L in range(k):
+        for i in range(n - 1):
+            c, s = g[L, i]
+            Gt = np.eye(n)
+            Gt[i, i], Gt[i, i + 1], Gt[i + 1, i], Gt[i + 1, i + 1] = c, -s, s, c
+            D = D @ Gt
+    F = A.copy(order="F")
+    oracle.bulk_rmul_seq_fma(F, g)
+    P = A.copy(order="F")
+    oracle.bulk_rmul_seq(P, g)
+    assert np.abs(F - D).max() <= 1e-14 * np.abs(D).max() * 4
+    assert 0 < np.abs(F - P).max() <= 1e-14 * np.abs(D).max() * 4
+
+
 @pytest.mark.parametrize("n,k", [(4, 2), (8, 6), (10, 8), (34, 32), (100, 98), (2000, 64)])
 def test_wave_order_bitwise_equals_layer_major(n, k):
     """src/wave.jl applies the grid in anti-diagonal order; without contraction every
@@ -185,13 +210,14 @@ def test_qr_algorithm_apply_modes():
     ):
         n = d.shape[0]
         outs = []
-        for mode in (0, 1, 2):
+        for mode in (0, 1, 2, 3):
             dv, ev = d.copy(), e.copy()
             Q = np.asfortranarray(np.eye(n))
             res = oracle.qr_algorithm(dv, ev, Q, 16, apply_mode=mode)
             outs.append((res, dv, ev, Q))
-        (r0, dv0, ev0, Q0), (r1, dv1, ev1, Q1), (r2, dv2, ev2, Q2) = outs
-        assert r0 == r1 == r2
+        (r0, dv0, ev0, Q0), (r1, dv1, ev1, Q1), (r2, dv2, ev2, Q2), (r3, dv3, ev3, Q3) = outs
+        assert r0 == r1 == r2 == r3 and np.array_equal(dv0, dv3)
+        assert np.abs(Q0 - Q3).max() <= 1e-13
         assert np.array_equal(dv0, dv1) and np.array_equal(dv0, dv2)
         assert np.array_equal(Q0, Q1)
         if bitwise:
