@@ -13,7 +13,8 @@ Also reported in the same line:
 * cpu_baseline: the oracle's wave-order apply (a C restatement of src/wave.jl and
   src/bulk_givens.jl) timed on this host's cores on a row sample of the same workload;
 * extra: the other apply shapes (n = 2000 Float64/Float32, n = 16384 Float32), the plain
-  solve of the README example (n = 2000) beside the oracle's CPU solve, the
+  solve of the README example (n = 2000) beside the oracle's CPU solve, the single large
+  solve (n = 16384, max_layers = 64), the
   differentiated-solve workload (n = 4096: forward solve, backward, and their ratio), and
   the batched workload (4096 problems: solves/s against the per-problem GPU loop).
 
@@ -96,7 +97,7 @@ def time_apply(sep, h, torch, n, k, dtype, steps=5):
     return 6.0 * n * (n - 1) * k / (ms * 1e-3) / 1e9, ms
 
 
-def extras(sep, h, torch, np, cpu=True):
+def extras(sep, h, torch, np, cpu=True, large=True):
     """Secondary figures: other apply shapes, config-1 solve, the differentiated solve, the
     batched solve."""
     sys.path.insert(0, ROOT)
@@ -133,6 +134,26 @@ def extras(sep, h, torch, np, cpu=True):
         ex["solve_n2000_cpu_port_note"] = ("oracle qr_algorithm (C restatement, FMA build), "
                                            "1 thread, chase + layer-major apply")
         log(f"[bench] solve n=2000 on the CPU port: {ex['solve_n2000_cpu_port_s']} s")
+
+    # config 3: single large solve, n = 16384, max_layers = 64 (chase of block b + 1 beside
+    # the apply of block b)
+    if large:
+        n = 16384
+        d, e = sep.inputs.tridiag_linear(n, seed=7)
+        H = sep.SymTridiagonal(torch.from_numpy(d).cuda(), torch.from_numpy(e).cuda())
+        Q = torch.eye(n, dtype=torch.float64, device="cuda").t()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, conv, it, nrot = sep.qr_algorithm_(H, Q, 64, handle=h, return_rotations=True)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        ex["solve_n16384_s"] = round(el, 3)
+        ex["solve_n16384_iter"] = it
+        ex["solve_n16384_rotations"] = nrot
+        ex["solve_n16384_chase_ns_per_rotation"] = round(el / nrot * 1e9, 1)
+        ex["solve_n16384_apply_gflops"] = round(6.0 * n * nrot / el / 1e9, 1)
+        log(f"[bench] solve n=16384: {el:.2f} s, iter={it}, rotations={nrot}")
+        del H, Q
 
     # config 5: differentiated solve, n = 4096, d_i = i, e ~ U[0.1, 1)
     n = 4096
@@ -221,6 +242,8 @@ def main() -> int:
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the n = 16384 solve in the extras (about 40 s)")
     ap.add_argument("--cpu-rows", type=int, default=16384)
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--calibrate-copy-gib", type=float, default=0.0,
@@ -327,7 +350,7 @@ def main() -> int:
         del src, dst
 
     if not args.no_extra:
-        out["extra"] = extras(sep, h, torch, np, cpu=not args.no_cpu)
+        out["extra"] = extras(sep, h, torch, np, cpu=not args.no_cpu, large=not args.no_large)
 
     print(json.dumps(out), flush=True)
     h.close()

@@ -584,6 +584,8 @@ int run_dpp(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, d
 
 using GeomDeep = Geom<4, 16, 8, 3>;   // 64 layers per pass, 4 waves per SIMD
 using GeomWide = Geom<8, 8, 16, 2>;   // 64 layers per pass, 2 waves per SIMD
+using GeomWide3 = Geom<8, 8, 16, 3>;  // as wide, Q prefetched three intervals ahead
+using GeomWide4 = Geom<8, 8, 16, 4>;  // as wide, Q prefetched four intervals ahead
 using GeomSmall = Geom<4, 8, 8, 3>;   // 32 layers per pass (short grids)
 
 }  // namespace
@@ -596,6 +598,9 @@ int bulk_apply_dpp_f64(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, i
     // 16 waves outweigh the higher occupancy)
     const int geom = h ? h->apply_geom : 0;
     if (geom == 1) return run_dpp<GeomDeep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (geom == 3) return run_dpp<GeomWide3>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (geom == 4) return run_dpp<GeomWide4>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (geom == 5) return run_dpp<GeomSmall>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
     if (k <= 32) return run_dpp<GeomSmall>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
     return run_dpp<GeomWide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
 }

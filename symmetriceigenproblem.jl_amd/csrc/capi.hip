@@ -240,7 +240,11 @@ int sep_create(sep_handle** out, int device)
     h->device = device;
     if (const char* ap = std::getenv("SEP_APPLY")) {
         h->force_sgpr_apply = std::strcmp(ap, "sgpr") == 0;
-        h->apply_geom = std::strcmp(ap, "deep") == 0 ? 1 : 0;
+        h->apply_geom = std::strcmp(ap, "deep") == 0    ? 1
+                        : std::strcmp(ap, "wide3") == 0 ? 3
+                        : std::strcmp(ap, "wide4") == 0 ? 4
+                        : std::strcmp(ap, "small") == 0 ? 5
+                                                        : 0;
     }
     // Blocking streams: work on them is ordered after earlier work on the legacy default
     // stream and before later work there, so device arrays a caller produced or consumes

@@ -199,6 +199,69 @@ __device__ __forceinline__ Rot<T> givens(T f, T g)
     return o;
 }
 
+__device__ __forceinline__ uint32_t hi_word(double x)
+{
+    return (uint32_t)((uint64_t)__double_as_longlong(x) >> 32);
+}
+// x with its sign bit xor-ed with the top bit of m (m is 0 or 0x80000000)
+__device__ __forceinline__ double xor_sign(double x, uint32_t m)
+{
+    return __longlong_as_double(__double_as_longlong(x) ^ (long long)((uint64_t)m << 32));
+}
+
+// Float64 specialisation of the same rotation with the bookkeeping moved off the vector
+// unit.  The values are wave-uniform, so:
+//  * the range test runs on the scalar unit from the high words: |x| < 2^485 exactly when
+//    hi(|x|) < hi(2^485), and hi(|x|) > hi(2^-380) implies |x| > 2^-380 (the few values
+//    just above 2^-380 that fail it take the complete path, which is exact for them too);
+//  * |f| > |g| is one vector compare read as a wave ballot, f < 0 is the sign bit;
+//  * the sign rule (c, s, r) -> -(c, s, r) is applied to r before dividing, since
+//    f / (-r) == -(f / r) exactly;
+//  * the reciprocal of r is refined from twice the square root's half-reciprocal estimate
+//    h ~ 1/(2 sqrt t) instead of a separate v_rcp_f64: two Newton steps from that seed
+//    reach RN(1/r) as the compiler's division sequence does, so the quotients are the
+//    same correctly rounded values, and the reciprocal no longer waits for r's final
+//    correction before it starts.
+template <>
+__device__ __forceinline__ Rot<double> givens<double>(double f, double g)
+{
+    const uint32_t fh = __builtin_amdgcn_readfirstlane((int)hi_word(f));
+    const uint32_t gh = __builtin_amdgcn_readfirstlane((int)hi_word(g));
+    const uint32_t fa = fh & 0x7fffffffu, ga = gh & 0x7fffffffu;
+    const uint32_t bigh = fa > ga ? fa : ga, smallh = fa < ga ? fa : ga;
+    const bool rare = !(bigh < 0x5E400000u) | !(smallh > 0x28300000u);  // 2^485, 2^-380
+    const bool gt = __ballot(tabs(f) > tabs(g)) != 0;
+    const uint32_t sm = (gt && (fh >> 31)) ? 0x80000000u : 0u;
+    // square root of t = f^2 + g^2 (as sqrt_common), keeping the half-reciprocal estimate
+    const double t = f * f + g * g;
+    const double y = __builtin_amdgcn_rsq(t);
+    double gq = t * y, h = 0.5 * y;
+    const double e = __builtin_fma(-h, gq, 0.5);
+    gq = __builtin_fma(gq, e, gq);
+    h = __builtin_fma(h, e, h);
+    const double y0 = xor_sign(h + h, sm);  // ~ 1/(+-r), available before r
+    double d = __builtin_fma(-gq, gq, t);
+    gq = __builtin_fma(d, h, gq);
+    d = __builtin_fma(-gq, gq, t);
+    gq = __builtin_fma(d, h, gq);
+    const double rr = xor_sign(gq, sm);  // +-RN(sqrt(t))
+    // two quotients over rr with the shared, Newton-refined reciprocal
+    double yy = y0;
+    double ee = __builtin_fma(-rr, yy, 1.0);
+    yy = __builtin_fma(yy, ee, yy);
+    ee = __builtin_fma(-rr, yy, 1.0);
+    yy = __builtin_fma(yy, ee, yy);
+    const double q1 = f * yy, q2 = g * yy;
+    const double r1 = __builtin_fma(-rr, q1, f), r2 = __builtin_fma(-rr, q2, g);
+    Rot<double> o;
+    o.c = __builtin_fma(r1, yy, q1);
+    o.s = __builtin_fma(r2, yy, q2);
+    o.r = rr;
+    asm volatile("" ::"v"(o.c), "v"(o.s), "v"(o.r));
+    if (__builtin_expect(rare, 0)) o = givens_slow(f, g);
+    return o;
+}
+
 // is_offdiagonal_small, src/bulge_chasing.jl:7-8 (1-based i; dv, ev 0-based arrays)
 template <typename T>
 __device__ __forceinline__ bool is_small(T e_i, T d_i, T d_ip1, T tol)
