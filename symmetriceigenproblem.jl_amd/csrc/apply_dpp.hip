@@ -584,8 +584,6 @@ int run_dpp(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, d
 
 using GeomDeep = Geom<4, 16, 8, 3>;   // 64 layers per pass, 4 waves per SIMD
 using GeomWide = Geom<8, 8, 16, 2>;   // 64 layers per pass, 2 waves per SIMD
-using GeomWide3 = Geom<8, 8, 16, 3>;  // as wide, Q prefetched three intervals ahead
-using GeomWide4 = Geom<8, 8, 16, 4>;  // as wide, Q prefetched four intervals ahead
 using GeomSmall = Geom<4, 8, 8, 3>;   // 32 layers per pass (short grids)
 
 }  // namespace
@@ -594,12 +592,12 @@ int bulk_apply_dpp_f64(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, i
                        double* A, int64_t lda, const double* cs, int64_t ld_cs, int64_t k,
                        bool rotations)
 {
-    // measured at m = n = 16384, k = 64: wide 2.70 ms, deep 2.92 ms (more barriers among
-    // 16 waves outweigh the higher occupancy)
+    // measured at m = n = 16384, k = 64 (kernel time per apply): wide 2.57 ms; deep 2.78 ms
+    // (more barriers among 16 waves outweigh the higher occupancy); wide with Q prefetched
+    // three intervals ahead 2.57 ms; two passes of 32 layers with two workgroups per CU
+    // 3.11 ms (KG 8, 4 waves) and 3.55 ms (KG 4, 8 waves)
     const int geom = h ? h->apply_geom : 0;
     if (geom == 1) return run_dpp<GeomDeep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if (geom == 3) return run_dpp<GeomWide3>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if (geom == 4) return run_dpp<GeomWide4>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
     if (geom == 5) return run_dpp<GeomSmall>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
     if (k <= 32) return run_dpp<GeomSmall>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
     return run_dpp<GeomWide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);

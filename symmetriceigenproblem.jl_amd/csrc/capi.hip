@@ -241,8 +241,6 @@ int sep_create(sep_handle** out, int device)
     if (const char* ap = std::getenv("SEP_APPLY")) {
         h->force_sgpr_apply = std::strcmp(ap, "sgpr") == 0;
         h->apply_geom = std::strcmp(ap, "deep") == 0    ? 1
-                        : std::strcmp(ap, "wide3") == 0 ? 3
-                        : std::strcmp(ap, "wide4") == 0 ? 4
                         : std::strcmp(ap, "small") == 0 ? 5
                                                         : 0;
     }

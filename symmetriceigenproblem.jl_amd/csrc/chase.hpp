@@ -311,16 +311,29 @@ __device__ __forceinline__ int64_t grid_pair(const Problem<T>& P, int i, int lay
     return (int64_t)(i - 1) + P.ldg * (layer - 1);
 }
 
+// 64-entry groups whose loads are in flight together: the deflation scan usually stops
+// within its first groups, the thickness maximum always reads its whole range
+constexpr int kScanBatch = 2;
+constexpr int kMaxBatch = 8;
+
 // The highest 1-based q < to with is_small(q), scanning down to `start`, or start-1.
+// Each round issues the loads of kScanBatch groups of 64 indices before testing any.
 template <typename T>
 __device__ int deflation_scan(const Problem<T>& P, int to, int start, T tol, int lane)
 {
-    for (int top = to - 1; top >= start; top -= kWave) {
-        const int q = top - lane;
-        bool sm = false;
-        if (q >= start) sm = is_small(P.ev[q - 1], P.dv[q - 1], P.dv[q], tol);
-        const uint64_t mask = __ballot(sm);
-        if (mask) return top - (int)__builtin_ctzll(mask);
+    for (int top = to - 1; top >= start; top -= kWave * kScanBatch) {
+        bool sm[kScanBatch];
+#pragma unroll
+        for (int u = 0; u < kScanBatch; ++u) {
+            const int q = top - kWave * u - lane;
+            sm[u] = false;
+            if (q >= start) sm[u] = is_small(P.ev[q - 1], P.dv[q - 1], P.dv[q], tol);
+        }
+#pragma unroll
+        for (int u = 0; u < kScanBatch; ++u) {
+            const uint64_t mask = __ballot(sm[u]);
+            if (mask) return top - kWave * u - (int)__builtin_ctzll(mask);
+        }
     }
     return start - 1;
 }
@@ -332,11 +345,21 @@ __device__ __forceinline__ int wave_max(int v)
     return v;
 }
 
+// max(layer_thickness[from .. to]) (1-based, inclusive), kMaxBatch groups per round trip.
 template <typename T>
 __device__ int max_thickness(const Problem<T>& P, int from, int to, int lane)
 {
     int v = 0;
-    for (int q = from + lane; q <= to; q += kWave) v = max(v, P.lt[q - 1]);
+    for (int q0 = from; q0 <= to; q0 += kWave * kMaxBatch) {
+        int w[kMaxBatch];
+#pragma unroll
+        for (int u = 0; u < kMaxBatch; ++u) {
+            const int q = q0 + kWave * u + lane;
+            w[u] = q <= to ? P.lt[q - 1] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kMaxBatch; ++u) v = max(v, w[u]);
+    }
     return wave_max(v);
 }
 
