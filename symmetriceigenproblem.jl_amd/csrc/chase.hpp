@@ -435,8 +435,9 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                 }
             }
             const int cnt = __builtin_amdgcn_readfirstlane(min(kWave, i_last - i0 + 1));
+            // outputs of step i0 + j are kept in lane j and stored once per chunk
+            T oR = T(0), oD = T(0), oC = T(0), oS = T(0);
             for (int j = 0; j < cnt; ++j) {
-                const int i = i0 + j;
                 const T B22 = rdlane(D, j);
                 const Rot<T> G = givens(p1, bulge);
                 const T c = G.c, s = G.s;
@@ -448,19 +449,28 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                 const T B11pp = __builtin_fma(B11p, c, B12p * s);
                 const T B12pp = __builtin_fma(B11p, -s, B12p * c);
                 const T B22pp = __builtin_fma(B21p, -s, B22p * c);
-                const int layer = rdlane(Lt, j) + 1;
-                // uniform stores of the finished outputs of step i
-                ev[i - 2] = G.r;   // ev[i-1] (1-based) = nrm
-                dv[i - 1] = B11pp; // dv[i]   (1-based) = B11''
-                lt[i - 1] = layer;
-                store_pair(grid + 2 * grid_pair(P, i, layer), c, s);
+                const bool mine = lane == j;
+                oR = mine ? G.r : oR;
+                oD = mine ? B11pp : oD;
+                oC = mine ? c : oC;
+                oS = mine ? s : oS;
                 p1 = B12pp;
                 d_cur = B22pp;
-                if (i + 2 <= to) {
+                if (i0 + j + 2 <= to) {
                     const T B23 = rdlane(E, j);
                     bulge = s * B23;
                     e_cur = c * B23;
                 }
+            }
+            // coalesced stores of the chunk; the next chunk was prefetched from entries
+            // these stores do not touch (dv/ev beyond i0 + 62, lt beyond i0 + 62)
+            if (lane < cnt) {
+                const int i = i0 + lane;
+                const int layer = Lt + 1;
+                ev[i - 2] = oR;   // ev[i-1] (1-based) = nrm
+                dv[i - 1] = oD;   // dv[i]   (1-based) = B11''
+                lt[i - 1] = layer;
+                store_pair(grid + 2 * grid_pair(P, i, layer), oC, oS);
             }
         }
     }
