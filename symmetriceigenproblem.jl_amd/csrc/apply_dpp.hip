@@ -13,12 +13,15 @@
 //    row, loaded with ordinary vector loads one whole interval ahead, and every FMA takes
 //    its coefficient through the DPP64 `row_newbcast:j` source modifier, which broadcasts
 //    lane j of each row.
-//  * A single wave issues FP64 FMAs at well under the SIMD's rate (measured: ~43% of peak
-//    with one wave per SIMD, ~70% with two, ~76% with four, ~95% with sixteen).  Two
-//    geometries are built: "wide" (G = 8 waves of KG = 8 layers, S = 16 steps per barrier
-//    interval, two waves per SIMD) and "deep" (G = 16 waves of KG = 4 layers, S = 8, four
-//    waves per SIMD); wide is the default because deep's doubled barrier count costs more
-//    than its occupancy gains.  The first wave prefetches Q two or three intervals ahead.
+//  * The row_newbcast operand costs issue rate: straight-line FP64 FMAs with a plain VGPR
+//    operand issue at ~96% of peak from one wave per SIMD, with the DPP operand at 67%
+//    (one wave per SIMD), 80% (two), 87% (four) and 94% (eight).  Two 64-row geometries
+//    are built: "wide" (G = 8 waves of KG = 8 layers, S = 16 steps per barrier interval,
+//    two waves per SIMD) and "deep" (G = 16 waves of KG = 4 layers, S = 8, four waves per
+//    SIMD); wide is used for large matrices because deep's doubled barrier count costs
+//    more than its occupancy gains.  The first wave prefetches Q two or three intervals
+//    ahead.  Below 64 * 256 rows these leave CUs idle; the 16- and 32-row geometries
+//    further down (one layer group per DPP row, several workgroups per CU) take over.
 //  * Only the middle waves load coefficients (for every wave, one interval ahead, staged
 //    through LDS); the first wave issues only Q loads and the last only Q stores, so no
 //    wave ever has to wait for a memory operation it issued for a later interval.
@@ -193,12 +196,14 @@ __global__ void __launch_bounds__(256) k_coef_table(double* __restrict__ tab,
           "+v"(SLOT(2, 0)), "+v"(SLOT(2, 1)), "+v"(SLOT(3, 0)), "+v"(SLOT(3, 1))               \
         : "v"(cf))
 
-// Shear step: KG rotations, coefficients in lanes base .. base + 2KG - 1.
-template <int KG, int s, int base>
-__device__ __forceinline__ void step_shear(double (&w)[2 * KG], double cf)
+// Shear step: KG rotations, coefficients in lanes base .. base + 2KG - 1.  The window is a
+// ring of WR >= 2KG registers; the column pair of layer l at step s is slots s - 2l and
+// s - 2l + 1 (mod WR).
+template <int KG, int s, int base, int WR>
+__device__ __forceinline__ void step_shear(double (&w)[WR], double cf)
 {
-    constexpr int W = 2 * KG;
-#define SLOT(l, o) w[(((s) - 2 * (l) + (o)) % W + W) % W]
+    static_assert(WR >= 2 * KG, "window ring too small");
+#define SLOT(l, o) w[(((s) - 2 * (l) + (o)) % WR + WR) % WR]
     if constexpr (KG == 8) {
         static_assert(base == 0, "KG = 8 fills a whole coefficient row");
         SEP_SHEAR8;
@@ -235,11 +240,11 @@ __device__ __forceinline__ void gen2(double& a0, double& b0, double& a1, double&
     else SEP_GEN2(12, 13, 14, 15);
 }
 
-template <int KG, int s, int base>
-__device__ __forceinline__ void step_general(double (&w)[2 * KG], double cf)
+template <int KG, int s, int base, int WR>
+__device__ __forceinline__ void step_general(double (&w)[WR], double cf)
 {
-    constexpr int W = 2 * KG;
-#define SLOT(l, o) w[(((s) - 2 * (l) + (o)) % W + W) % W]
+    static_assert(WR >= 2 * KG, "window ring too small");
+#define SLOT(l, o) w[(((s) - 2 * (l) + (o)) % WR + WR) % WR]
     gen2<base / 4 + 0>(SLOT(0, 0), SLOT(0, 1), SLOT(1, 0), SLOT(1, 1), cf);
     gen2<base / 4 + 1>(SLOT(2, 0), SLOT(2, 1), SLOT(3, 0), SLOT(3, 1), cf);
     if constexpr (KG == 8) {
@@ -586,20 +591,351 @@ using GeomDeep = Geom<4, 16, 8, 3>;   // 64 layers per pass, 4 waves per SIMD
 using GeomWide = Geom<8, 8, 16, 2>;   // 64 layers per pass, 2 waves per SIMD
 using GeomSmall = Geom<4, 8, 8, 3>;   // 32 layers per pass (short grids)
 
+// ---- 16-row geometry (R16_BEGIN) ---------------------------------------------------
+// One workgroup owns 16 rows of A and all Kp = 4 NW KG layers of a pass.  Lane 16 r + j
+// of wave w holds row j of the block in layer group q = 4 w + r (layers q KG .. q KG +
+// KG - 1): every 16-lane DPP row is a different layer group, and the row_newbcast
+// coefficient broadcast gives each group its own rotations.  Groups hand columns on
+// through LDS: in interval b (S steps) every lane writes the column leaving its window to
+// out[w][b & 1][s][lane]; in interval b + 1 lane 16 r + j (r > 0) reads the slot of lane
+// 16 (r - 1) + j of its own wave, and lane j (r = 0) reads lane 48 + j of the previous
+// wave's buffer (wave 0: a staging buffer that wave 0 fills with columns of A).  Each
+// group therefore runs S - 1 time steps behind the previous one: group q at local step u
+// applies rotation i = u - q (S - 1) - 2 (q KG + l) of its layer l, and the last group
+// hands out column u - DL.  Every wave runs the same FMA stream with no bounds tests
+// (columns outside the matrix are zeros under identity rotations); wave 0 additionally
+// loads A four columns per instruction (lane 16 c + j: column c, row j) and the last wave
+// stores, sign-fixes and reads back its outputs the same way.  With 16 rows per
+// workgroup, four independent workgroups share a CU, so the waves sharing a SIMD belong
+// to different workgroups and do not stall at the same barriers.
+constexpr int gcd_(int a, int b) { return b == 0 ? a : gcd_(b, a % b); }
+constexpr int lcm_(int a, int b) { return a / gcd_(a, b) * b; }
+
+template <int KG_, int NW_, int S_, int RPG_ = 16>
+struct G16 {
+    static constexpr int KG = KG_, NW = NW_, S = S_;
+    static constexpr int RPG = RPG_;           // rows of A per block = lanes per layer group
+    static constexpr int GPW = kLanes / RPG;   // layer groups per wave
+    static constexpr int OL = kLanes - RPG;    // first lane of a wave's last group
+    static constexpr int NG = GPW * NW;        // layer groups
+    static constexpr int Kp = NG * KG;  // layers per pass
+    static constexpr int W = 2 * KG;    // window columns
+    static constexpr int NC = 2 * KG;   // coefficients per step and group
+    static constexpr int SPR = 16 / NC; // steps per coefficient register
+    static constexpr int NR = S / SPR;  // coefficient registers per interval
+    static constexpr int NQ = S / GPW;  // GPW-column loads / stores per interval
+    static constexpr int DL = (NG - 1) * (S - 1) + 2 * Kp - 2;  // output column = u - DL
+    static constexpr int WR = W + S;    // window ring: an interval's inputs load into free slots
+    static constexpr int U = lcm_(WR, 2 * S) / S;  // intervals per unrolled loop body (even)
+    static_assert(S % GPW == 0 && S % SPR == 0 && U % 2 == 0, "bad interval length");
+    static_assert(RPG == 16 || RPG == 32, "a layer group is one or two DPP rows");
+};
+
+// Per-wave constants of the 16-row kernel.
+struct R16Args {
+    double* A0;            // A + first row of the block (uniform)
+    int64_t voff;          // lane offset: (lane / RPG) lda + row within the block
+    int64_t lda;
+    const double* tab;
+    const double* sigm1;
+    int n, NBt, w, lane;
+    bool first, last;
+};
+
+// tab[((w NBt + b) NR + rr) 64 + lane]: the coefficient register rr of interval b of wave
+// w.  Lane 16 r + j of it belongs to group q = 4 w + r, step u = b S + rr SPR + j / NC,
+// layer q KG + (j % NC) / 2; even j hold p (shear) or c - 1 (general), odd j hold q or s.
+template <class Gm>
+__global__ void __launch_bounds__(256) k_coef_r16(double* __restrict__ tab,
+                                                  const double* __restrict__ cs, int64_t ld,
+                                                  int n, int k, int L0, int NBt,
+                                                  const int* __restrict__ flag,
+                                                  const uint64_t* __restrict__ mask)
+{
+    constexpr int S = Gm::S, KG = Gm::KG, NC = Gm::NC, SPR = Gm::SPR, NR = Gm::NR;
+    const int64_t total = (int64_t)Gm::NW * NBt * NR * kLanes;
+    const int general = *flag;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int lane = (int)(e % kLanes);
+        const int64_t rest = e / kLanes;
+        const int rr = (int)(rest % NR);
+        const int64_t wb = rest / NR;
+        const int b = (int)(wb % NBt);
+        const int w = (int)(wb / NBt);
+        const int j = lane % 16;  // coefficient slot within the DPP row
+        const int q = Gm::GPW * w + lane / Gm::RPG;
+        const int64_t u = (int64_t)b * S + rr * SPR + j / NC;
+        const int lr = q * KG + (j % NC) / 2;
+        const int L = L0 + lr;
+        const int64_t i = u - (int64_t)q * (S - 1) - 2 * lr;
+        double v = 0.0;
+        if (L < k && i >= 0 && i <= n - 2) {
+            const double c = cs[2 * (i + ld * L)], s = cs[2 * (i + ld * L) + 1];
+            if (general) {
+                v = (j & 1) ? s : c - 1.0;
+            } else {
+                // stored signs of columns i and i+1 when rotation (i, L) is applied
+                bool na = (mask[i] >> lr) & 1;
+                na ^= neg_c(cs, ld, n, i - 1, L, k);
+                const bool nb = (mask[i + 1] >> lr) & 1;
+                const double st = ((c < 0.0) ^ na ^ nb) ? -s : s;
+                v = (j & 1) ? -st : st / (1.0 + fabs(c));
+            }
+        }
+        tab[e] = v;
+    }
+}
+
+// One interval (S steps) of every wave.  IU is the interval's index in the unrolled loop
+// body: its parity P selects the LDS buffers and its first local step fixes the window
+// slots, which are compile-time registers (the same for every lane).
+template <class Gm, bool GENERAL, int IU>
+__device__ __forceinline__ void r16_interval(const R16Args& a, double (*buf)[2][Gm::S][kLanes],
+                                             double (&win)[Gm::WR], double (&cf)[2][Gm::NR],
+                                             double (&qv)[2][Gm::NQ], const double* rd,
+                                             double* wr, int b)
+{
+    constexpr int S = Gm::S, WR = Gm::WR, KG = Gm::KG, NW = Gm::NW, NR = Gm::NR;
+    constexpr int NQ = Gm::NQ, SPR = Gm::SPR, NC = Gm::NC, W = Gm::W;
+    constexpr int P = IU & 1, s0 = IU * S;
+    constexpr int RPG = Gm::RPG, GPW = Gm::GPW, OL = Gm::OL;
+    const int lane = a.lane, r = lane / RPG, j = lane % RPG;
+    // inputs of this interval (written in interval b - 1) go straight into their window
+    // slots: slot (u + 1) mod WR was last used by the column that left S + 1 steps ago
+#pragma unroll
+    for (int s = 0; s < S; ++s) win[(s0 + s + 1) % WR] = rd[((1 - P) * S + s) * kLanes];
+    // last wave: its outputs of interval b - 1, columns (b - 1) S + 4 k + r - DL
+    double ob[NQ], sg[NQ];
+    const int oc0 = (b - 1) * S - Gm::DL;  // output column of lane row 0, k = 0
+    const bool oful = oc0 >= 0 && oc0 + S <= a.n;
+    if (a.last) {
+#pragma unroll
+        for (int k4 = 0; k4 < NQ; ++k4) {
+            ob[k4] = buf[NW][1 - P][GPW * k4 + r][OL + j];
+            const int oc = oc0 + GPW * k4 + r;
+            sg[k4] = (oful || (oc >= 0 && oc < a.n)) ? a.sigm1[oc] : 0.0;
+        }
+    }
+    Steps<S, 0>::run([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        constexpr int sw = (s0 + s) % WR;
+        constexpr int rr = s / SPR, base = (s % SPR) * NC;
+        if constexpr (GENERAL)
+            step_general<KG, sw, base>(win, cf[P][rr]);
+        else
+            step_shear<KG, sw, base>(win, cf[P][rr]);
+        // the column leaving the window entered W - 1 steps ago
+        wr[(P * S + s) * kLanes] = win[(sw + 2 - W + WR) % WR];
+    });
+    // coefficients of interval b + 2 (the table has NB + 2 intervals)
+    const double* __restrict__ tb = a.tab + ((int64_t)(a.w * a.NBt + b + 2) * NR) * kLanes;
+#pragma unroll
+    for (int rr = 0; rr < NR; ++rr) cf[P][rr] = tb[rr * kLanes + lane];
+    if (a.first) {
+        // staging for interval b + 1 (columns (b + 1) S + 1 + s), then the columns of the
+        // staging written in interval b + 2
+#pragma unroll
+        for (int k4 = 0; k4 < NQ; ++k4) buf[0][P][GPW * k4 + r][OL + j] = qv[P][k4];
+        const int c0 = (b + 3) * S + 1;
+        const double* __restrict__ src = a.A0 + (int64_t)c0 * a.lda;
+        if (c0 + S <= a.n) {
+#pragma unroll
+            for (int k4 = 0; k4 < NQ; ++k4) qv[P][k4] = src[(int64_t)GPW * k4 * a.lda + a.voff];
+        } else {
+#pragma unroll
+            for (int k4 = 0; k4 < NQ; ++k4)
+                qv[P][k4] = c0 + GPW * k4 + r < a.n ? src[(int64_t)GPW * k4 * a.lda + a.voff] : 0.0;
+        }
+    }
+    if (a.last) {
+        double* __restrict__ dst = a.A0 + (int64_t)oc0 * a.lda;
+        if (oful) {
+#pragma unroll
+            for (int k4 = 0; k4 < NQ; ++k4)
+                dst[(int64_t)GPW * k4 * a.lda + a.voff] = fma(sg[k4], ob[k4], ob[k4]);
+        } else {
+#pragma unroll
+            for (int k4 = 0; k4 < NQ; ++k4) {
+                const int oc = oc0 + GPW * k4 + r;
+                if (oc >= 0 && oc < a.n)
+                    dst[(int64_t)GPW * k4 * a.lda + a.voff] = fma(sg[k4], ob[k4], ob[k4]);
+            }
+        }
+    }
+    lds_barrier();
+}
+
+template <class Gm, bool GENERAL>
+__device__ __forceinline__ void r16_body(double* __restrict__ A, int64_t lda, int m, int n,
+                                         const double* __restrict__ tab,
+                                         const double* __restrict__ sigm1, int NB, int NBt,
+                                         double (*buf)[2][Gm::S][kLanes])
+{
+    constexpr int S = Gm::S, WR = Gm::WR, NW = Gm::NW, NR = Gm::NR, NQ = Gm::NQ, U = Gm::U;
+    R16Args a;
+    a.w = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
+    a.lane = threadIdx.x % kLanes;
+    a.first = a.w == 0;
+    a.last = a.w == NW - 1;
+    a.n = n;
+    a.lda = lda;
+    a.NBt = NBt;
+    a.tab = tab;
+    a.sigm1 = sigm1;
+    constexpr int RPG = Gm::RPG, GPW = Gm::GPW, OL = Gm::OL;
+    const int r = a.lane / RPG, j = a.lane % RPG;
+    const int64_t row0 = (int64_t)blockIdx.x * RPG;
+    // Rows past m alias row m - 1: those lanes compute and store that row's own values.
+    const int jr = (int)(min(row0 + j, (int64_t)m - 1) - row0);
+    a.A0 = A + row0;                          // uniform
+    a.voff = (int64_t)r * lda + jr;           // lane RPG c + j: column c, row j
+    const double* rd = r == 0 ? &buf[a.w][0][0][OL + j] : &buf[a.w + 1][0][0][a.lane - RPG];
+    double* wr = &buf[a.w + 1][0][0][a.lane];
+
+    double win[WR];
+#pragma unroll
+    for (int s = 0; s < WR; ++s) win[s] = 0.0;
+    // interval 0 reads parity 1: zero every wave's buffer, stage columns 1 .. S
+#pragma unroll
+    for (int s = 0; s < S; ++s) wr[(S + s) * kLanes] = 0.0;
+    double qv[2][NQ];
+    if (a.first) {
+        if (r == 0) win[0] = A[row0 + jr];  // column 0 sits in slot 0 before step 0
+#pragma unroll
+        for (int k4 = 0; k4 < NQ; ++k4) {
+            const int c0 = 1 + GPW * k4 + r, c1 = S + c0, c2 = 2 * S + c0;
+            const double* __restrict__ p = a.A0 + (int64_t)(GPW * k4) * lda + a.voff;
+            buf[0][1][GPW * k4 + r][OL + j] = c0 < n ? p[(int64_t)1 * lda] : 0.0;
+            qv[0][k4] = c1 < n ? p[(int64_t)(S + 1) * lda] : 0.0;
+            qv[1][k4] = c2 < n ? p[(int64_t)(2 * S + 1) * lda] : 0.0;
+        }
+    }
+    double cf[2][NR];
+#pragma unroll
+    for (int rr = 0; rr < NR; ++rr) {
+        cf[0][rr] = tab[((int64_t)(a.w * NBt + 0) * NR + rr) * kLanes + a.lane];
+        cf[1][rr] = tab[((int64_t)(a.w * NBt + 1) * NR + rr) * kLanes + a.lane];
+    }
+    lds_barrier();
+    for (int b = 0; b < NB; b += U) {
+        Steps<U, 0>::run([&](auto ic) {
+            constexpr int IU = decltype(ic)::value;
+            r16_interval<Gm, GENERAL, IU>(a, buf, win, cf, qv, rd, wr, b + IU);
+        });
+    }
+    // outputs of the last interval (NB - 1, parity 1)
+    if (a.last) {
+#pragma unroll
+        for (int k4 = 0; k4 < NQ; ++k4) {
+            const int oc = (NB - 1) * S + GPW * k4 + r - Gm::DL;
+            if (oc >= 0 && oc < n) {
+                const double x = buf[NW][1][GPW * k4 + r][OL + j];
+                a.A0[(int64_t)(oc - r) * lda + a.voff] = fma(sigm1[oc], x, x);
+            }
+        }
+    }
+}
+
+template <class Gm>
+__global__ void __launch_bounds__(kLanes * Gm::NW)
+    k_apply_r16(double* __restrict__ A, int64_t lda, int m, int n,
+                const double* __restrict__ tab, const int* __restrict__ flag,
+                const double* __restrict__ sigm1, int NB, int NBt)
+{
+    __shared__ double buf[Gm::NW + 1][2][Gm::S][kLanes];
+    if (*flag != 0)
+        r16_body<Gm, true>(A, lda, m, n, tab, sigm1, NB, NBt, buf);
+    else
+        r16_body<Gm, false>(A, lda, m, n, tab, sigm1, NB, NBt, buf);
+}
+// (R16_END)
+
+template <class Gm>
+int run_r16(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, double* A,
+            int64_t lda, const double* cs, int64_t ld_cs, int64_t k, bool rotations)
+{
+    constexpr int S = Gm::S, Kp = Gm::Kp;
+    int NB = (int)((n + Gm::DL + S - 1) / S);
+    NB = (NB + Gm::U - 1) / Gm::U * Gm::U;  // whole unrolled loop bodies
+    const int NBt = NB + 2;
+    const size_t tab_elems = (size_t)Gm::NW * NBt * Gm::NR * kLanes;
+    if (int rc = w.table.reserve(sizeof(double) * tab_elems)) return rc;
+    if (int rc = w.aux.reserve(sizeof(uint64_t) * (size_t)n + sizeof(double) * (size_t)n + 256))
+        return rc;
+    uint64_t* mask = w.aux.as<uint64_t>();
+    double* sigm1 = reinterpret_cast<double*>(mask + n);
+    int* flag = reinterpret_cast<int*>(sigm1 + n);
+    const int tp0 = timing_mark(h, st);
+    SEP_HIP(hipMemsetAsync(flag, 0, sizeof(int), st));
+    if (!rotations) {
+        const int64_t tot = (n - 1) * k;
+        const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 2048);
+        hipLaunchKernelGGL(k_check_norm, dim3(blocks), dim3(256), 0, st, cs, ld_cs, (int)n,
+                           (int)k, flag);
+        SEP_CHECK_LAUNCH();
+    }
+    const int64_t wgs = (m + Gm::RPG - 1) / Gm::RPG;
+    int tp_prev = tp0;
+    for (int64_t L0 = 0; L0 < k; L0 += Kp) {
+        hipLaunchKernelGGL(k_colsigns, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cs,
+                           ld_cs, (int)n, (int)k, (int)L0, Kp, flag, mask, sigm1);
+        SEP_CHECK_LAUNCH();
+        const int tblocks = (int)std::min<int64_t>(((int64_t)tab_elems + 255) / 256, 8192);
+        hipLaunchKernelGGL((k_coef_r16<Gm>), dim3(tblocks), dim3(256), 0, st,
+                           w.table.as<double>(), cs, ld_cs, (int)n, (int)k, (int)L0, NBt, flag,
+                           mask);
+        SEP_CHECK_LAUNCH();
+        const int ta = timing_mark(h, st);
+        timing_pair(h, false, tp_prev, ta);
+        hipLaunchKernelGGL((k_apply_r16<Gm>), dim3((unsigned)wgs), dim3(kLanes * Gm::NW), 0, st,
+                           A, lda, (int)m, (int)n, w.table.as<double>(), flag, sigm1, NB, NBt);
+        SEP_CHECK_LAUNCH();
+        tp_prev = timing_mark(h, st);
+        timing_pair(h, true, ta, tp_prev);
+    }
+    return SEP_OK;
+}
+
+using G16Wide = G16<8, 2, 8>;   // 64 layers per pass: 2 waves of 4 groups of 8 layers
+using G16Deep = G16<4, 4, 8>;   // 64 layers per pass: 4 waves of 4 groups of 4 layers
+using G16Small = G16<4, 2, 8>;  // 32 layers per pass
+using G32Wide = G16<8, 4, 16, 32>;  // 32-row blocks: 4 waves of 2 groups of 8 layers
+using G32Deep = G16<4, 8, 8, 32>;   // 32-row blocks: 8 waves of 2 groups of 4 layers
+
 }  // namespace
 
 int bulk_apply_dpp_f64(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n,
                        double* A, int64_t lda, const double* cs, int64_t ld_cs, int64_t k,
                        bool rotations)
 {
-    // measured at m = n = 16384, k = 64 (kernel time per apply): wide 2.57 ms; deep 2.78 ms
-    // (more barriers among 16 waves outweigh the higher occupancy); wide with Q prefetched
-    // three intervals ahead 2.57 ms; two passes of 32 layers with two workgroups per CU
-    // 3.11 ms (KG 8, 4 waves) and 3.55 ms (KG 4, 8 waves)
+    // Measured kernel time per apply, m = n, k = 64 (ms):
+    //            wide   deep   r16    r16deep r32    r32deep
+    //   2000     0.304  0.317  0.200  0.192   0.227  0.259
+    //   4096     0.589  0.611  0.383  0.373   0.429  0.494
+    //   8192     1.181  1.203  1.076  0.987   0.887  1.199
+    //   12000    1.824  1.766  2.169  1.715   1.916  2.133
+    //   16384    2.737  2.825  3.445  3.026   2.913  3.332
+    // The 64-row geometries leave CUs idle below m = 64 * 256 rows; the 16- and 32-row ones
+    // fill them but cost more per row (per-workgroup coefficient traffic, and workgroups
+    // that drift apart: at m = 16384 r16's workgroups take 1.9 to 3.4 ms).  Also measured
+    // at 16384: wide with Q prefetched three intervals ahead 2.57 ms; two passes of 32
+    // layers with two workgroups per CU 3.11 ms (KG 8, 4 waves) and 3.55 ms (KG 4, 8
+    // waves); neighbour-to-neighbour interval counters in LDS instead of the workgroup
+    // barrier 2.66 ms.
     const int geom = h ? h->apply_geom : 0;
+    if (geom == 2) return run_dpp<GeomWide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
     if (geom == 1) return run_dpp<GeomDeep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
     if (geom == 5) return run_dpp<GeomSmall>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (geom == 6) return run_r16<G16Wide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (geom == 7) return run_r16<G16Deep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (geom == 8) return run_r16<G16Small>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (geom == 9) return run_r16<G32Wide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (geom == 10) return run_r16<G32Deep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
     if (k <= 32) return run_dpp<GeomSmall>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (m <= 6144 || (m > 10240 && m <= 14336))
+        return run_r16<G16Deep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (m <= 10240) return run_r16<G32Wide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
     return run_dpp<GeomWide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
 }
 

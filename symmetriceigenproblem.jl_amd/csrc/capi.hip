@@ -240,9 +240,12 @@ int sep_create(sep_handle** out, int device)
     h->device = device;
     if (const char* ap = std::getenv("SEP_APPLY")) {
         h->force_sgpr_apply = std::strcmp(ap, "sgpr") == 0;
-        h->apply_geom = std::strcmp(ap, "deep") == 0    ? 1
-                        : std::strcmp(ap, "small") == 0 ? 5
-                                                        : 0;
+        // Float64 apply geometry (unset: chosen by the number of rows)
+        static const struct { const char* name; int geom; } kGeoms[] = {
+            {"deep", 1},    {"wide", 2},     {"small", 5}, {"r16", 6},
+            {"r16deep", 7}, {"r16small", 8}, {"r32", 9},   {"r32deep", 10}};
+        for (const auto& g : kGeoms)
+            if (std::strcmp(ap, g.name) == 0) h->apply_geom = g.geom;
     }
     // Blocking streams: work on them is ordered after earlier work on the legacy default
     // stream and before later work there, so device arrays a caller produced or consumes

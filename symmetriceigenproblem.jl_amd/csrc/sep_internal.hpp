@@ -97,7 +97,7 @@ struct sep_handle {
     int tune_choice = -1;
     int force_kf = 0, force_r = 0;  // sep_set_batched_config (0: autotuned / default)
     bool force_sgpr_apply = false;  // SEP_APPLY=sgpr selects the scalar-fed Float64 kernel
-    int apply_geom = 0;  // SEP_APPLY=deep / small select the 16-wave or the 32-layer geometry
+    int apply_geom = 0;  // Float64 apply geometry forced by SEP_APPLY (0: chosen by size)
     // kernel timing (sep_set_timing): event pairs recorded around apply / prep kernels
     bool timing = false;
     std::vector<hipEvent_t> tev;            // pool, reused
