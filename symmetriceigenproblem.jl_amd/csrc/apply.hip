@@ -23,6 +23,9 @@
 //    order (per wave, per time step, KG contiguous pairs, identity padding outside
 //    the grid), which the main kernel reads with scalar loads into SGPRs.  The inner
 //    loop is then 4 VALU ops per rotation and row (2 mul + 2 fma) and nothing else.
+//  * Both element types now go to the DPP-fed kernels of apply_dpp.hip by default (2.9x
+//    this kernel's Float32 rate at n = 16384); these kernels stay selectable with
+//    SEP_APPLY=sgpr.
 #include <type_traits>
 
 #include "sep_internal.hpp"
@@ -260,10 +263,8 @@ int bulk_apply_dev(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64
                   (long long)m, (long long)n);
         return SEP_ERR_ARG;
     }
-    if constexpr (std::is_same<T, double>::value) {
-        if (h == nullptr || !h->force_sgpr_apply)
-            return bulk_apply_dpp_f64(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    }
+    if (h == nullptr || !h->force_sgpr_apply)
+        return bulk_apply_dpp<T>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
     if (k <= 32) return run_passes<T, 8, 4, 16>(h, st, w, m, n, A, lda, cs, ld_cs, k);
     return run_passes<T, 8, 8, 16>(h, st, w, m, n, A, lda, cs, ld_cs, k);
 }

@@ -1,4 +1,5 @@
-// apply_dpp.hip -- Float64 bulk Givens application with DPP-broadcast coefficients.
+// apply_dpp.hip -- bulk Givens application (Float64 and Float32) with DPP-broadcast
+// coefficients.
 //
 // Same product as bulk_wave_order_2x2_rmul! (src/wave.jl:62-162): A <- A * prod_L prod_i
 // G(i,L)^T, each rotation a' = c a + s b, b' = c b - s a on the column pair (i, i+1)
@@ -47,7 +48,8 @@ namespace {
 constexpr int kLanes = 64;
 
 // flag = 1 if any rotation of the grid is not normalised.
-__global__ void __launch_bounds__(256) k_check_norm(const double* __restrict__ cs, int64_t ld,
+template <typename T>
+__global__ void __launch_bounds__(256) k_check_norm(const T* __restrict__ cs, int64_t ld,
                                                     int n, int k, int* __restrict__ flag)
 {
     const int64_t total = (int64_t)(n - 1) * k;
@@ -55,13 +57,14 @@ __global__ void __launch_bounds__(256) k_check_norm(const double* __restrict__ c
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
          q += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = q % (n - 1), L = q / (n - 1);
-        const double c = cs[2 * (i + ld * L)], s = cs[2 * (i + ld * L) + 1];
-        bad |= !(fabs(c * c + s * s - 1.0) <= 1e-14);
+        const T c = cs[2 * (i + ld * L)], s = cs[2 * (i + ld * L) + 1];
+        bad |= !(fabs(c * c + s * s - T(1)) <= (sizeof(T) == 8 ? T(1e-14) : T(1e-6)));
     }
     if (__any(bad) && (threadIdx.x % kLanes) == 0) atomicOr(flag, 1);
 }
 
-__device__ __forceinline__ bool neg_c(const double* __restrict__ cs, int64_t ld, int n, int64_t i,
+template <typename T>
+__device__ __forceinline__ bool neg_c(const T* __restrict__ cs, int64_t ld, int n, int64_t i,
                                       int L, int k)
 {
     if (i < 0 || i > n - 2 || L >= k) return false;
@@ -70,11 +73,12 @@ __device__ __forceinline__ bool neg_c(const double* __restrict__ cs, int64_t ld,
 
 // Per column j: bit lr of mask[j] = the stored sign of column j is negative before the
 // rotations of layer L0 + lr; sigm1[j] = (final sign) - 1, i.e. 0 or -2.
-__global__ void __launch_bounds__(256) k_colsigns(const double* __restrict__ cs, int64_t ld,
+template <typename T>
+__global__ void __launch_bounds__(256) k_colsigns(const T* __restrict__ cs, int64_t ld,
                                                   int n, int k, int L0, int Kp,
                                                   const int* __restrict__ flag,
                                                   uint64_t* __restrict__ mask,
-                                                  double* __restrict__ sigm1)
+                                                  T* __restrict__ sigm1)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
@@ -90,7 +94,7 @@ __global__ void __launch_bounds__(256) k_colsigns(const double* __restrict__ cs,
         if (L >= k) break;
         if (neg) m |= 1ull << lr;
         // rotations (j-1, L) then (j, L) touch column j in layer L
-        neg ^= neg_c(cs, ld, n, j - 1, L, k) ^ neg_c(cs, ld, n, j, L, k);
+        neg ^= neg_c<T>(cs, ld, n, j - 1, L, k) ^ neg_c<T>(cs, ld, n, j, L, k);
     }
     mask[j] = m;
     sigm1[j] = neg ? -2.0 : 0.0;
@@ -104,13 +108,13 @@ __global__ void __launch_bounds__(256) k_colsigns(const double* __restrict__ cs,
 // Section g = G holds, for every interval bt, the last wave's 16-lane output-sign row at
 // the position a coefficient row of a (virtual) wave G would have, so the coefficient
 // loaders fetch it with the same address pattern.
-template <int KG, int G, int S>
-__global__ void __launch_bounds__(256) k_coef_table(double* __restrict__ tab,
-                                                    const double* __restrict__ cs, int64_t ld,
+template <typename T, int KG, int G, int S>
+__global__ void __launch_bounds__(256) k_coef_table(T* __restrict__ tab,
+                                                    const T* __restrict__ cs, int64_t ld,
                                                     int n, int k, int L0, int Up, int PADU,
                                                     int NI, const int* __restrict__ flag,
                                                     const uint64_t* __restrict__ mask,
-                                                    const double* __restrict__ sigm1)
+                                                    const T* __restrict__ sigm1)
 {
     constexpr int NC = 2 * KG, Kp = KG * G;
     const int64_t rot_slots = (int64_t)G * Up * KG;
@@ -123,7 +127,7 @@ __global__ void __launch_bounds__(256) k_coef_table(double* __restrict__ tab,
             const int bt = qq / 16, c = qq % 16;
             const int tl = (bt - (G - 1)) * S + (G - 1);
             const int col = tl - 2 * Kp + 2 + c;
-            const double v = (c < S && col >= 0 && col < n && !general) ? sigm1[col] : 0.0;
+            const T v = (c < S && col >= 0 && col < n && !general) ? sigm1[col] : 0.0;
             tab[(int64_t)G * Up * NC + (int64_t)(bt * S + G * (1 - S) + S + PADU) * NC + c] = v;
             continue;
         }
@@ -134,18 +138,18 @@ __global__ void __launch_bounds__(256) k_coef_table(double* __restrict__ tab,
         const int lr = g * KG + l;
         const int L = L0 + lr;
         const int64_t i = (int64_t)(u - S - PADU) - 2 * (int64_t)lr;
-        double p = 0.0, qq = 0.0;
+        T p = 0.0, qq = 0.0;
         if (L < k && i >= 0 && i <= n - 2) {
-            const double c = cs[2 * (i + ld * L)], s = cs[2 * (i + ld * L) + 1];
+            const T c = cs[2 * (i + ld * L)], s = cs[2 * (i + ld * L) + 1];
             if (general) {
                 p = c - 1.0;
                 qq = s;
             } else {
                 bool na = (mask[i] >> lr) & 1;
-                na ^= neg_c(cs, ld, n, i - 1, L, k);
+                na ^= neg_c<T>(cs, ld, n, i - 1, L, k);
                 const bool nb = (mask[i + 1] >> lr) & 1;
-                const double ct = fabs(c);
-                const double st = ((c < 0.0) ^ na ^ nb) ? -s : s;
+                const T ct = fabs(c);
+                const T st = ((c < 0.0) ^ na ^ nb) ? -s : s;
                 p = st / (1.0 + ct);
                 qq = -st;
             }
@@ -161,7 +165,8 @@ __global__ void __launch_bounds__(256) k_coef_table(double* __restrict__ tab,
 // DPP row_newbcast sources.  The leading s_nop covers the VALU-write -> DPP-read hazard
 // in case the compiler copied the coefficient row with a VALU move right before the block.
 #define SEP_DPP(a, c, b, lane) \
-    "v_fmac_f64_dpp " a ", " c ", " b " row_newbcast:" #lane " row_mask:0xf bank_mask:0xf\n"
+    SEP_FMAC " " a ", " c ", " b " row_newbcast:" #lane " row_mask:0xf bank_mask:0xf\n"
+// SEP_FMAC names v_fmac_f64_dpp or v_fmac_f32_dpp; it is defined around each use below.
 #define SEP_ROT3(a, b, p, q) /* a += p b; b += q a; a += p b (three shears) */
 
 #define SEP_SHEAR8                                                                             \
@@ -199,21 +204,30 @@ __global__ void __launch_bounds__(256) k_coef_table(double* __restrict__ tab,
 // Shear step: KG rotations, coefficients in lanes base .. base + 2KG - 1.  The window is a
 // ring of WR >= 2KG registers; the column pair of layer l at step s is slots s - 2l and
 // s - 2l + 1 (mod WR).
-template <int KG, int s, int base, int WR>
-__device__ __forceinline__ void step_shear(double (&w)[WR], double cf)
+template <typename T, int KG, int s, int base, int WR>
+__device__ __forceinline__ void step_shear(T (&w)[WR], T cf)
 {
     static_assert(WR >= 2 * KG, "window ring too small");
+    static_assert(KG == 8 || KG == 4, "KG is 4 or 8");
+    static_assert(KG == 4 || base == 0, "KG = 8 fills a whole coefficient row");
 #define SLOT(l, o) w[(((s) - 2 * (l) + (o)) % WR + WR) % WR]
-    if constexpr (KG == 8) {
-        static_assert(base == 0, "KG = 8 fills a whole coefficient row");
-        SEP_SHEAR8;
+#define SEP_SHEAR_ANY                                                                       \
+    if constexpr (KG == 8)                                                                  \
+        SEP_SHEAR8;                                                                         \
+    else if constexpr (base == 0)                                                           \
+        SEP_SHEAR4(0, 1, 2, 3, 4, 5, 6, 7);                                                 \
+    else                                                                                    \
+        SEP_SHEAR4(8, 9, 10, 11, 12, 13, 14, 15);
+    if constexpr (sizeof(T) == 8) {
+#define SEP_FMAC "v_fmac_f64_dpp"
+        SEP_SHEAR_ANY
+#undef SEP_FMAC
     } else {
-        static_assert(KG == 4, "KG is 4 or 8");
-        if constexpr (base == 0)
-            SEP_SHEAR4(0, 1, 2, 3, 4, 5, 6, 7);
-        else
-            SEP_SHEAR4(8, 9, 10, 11, 12, 13, 14, 15);
+#define SEP_FMAC "v_fmac_f32_dpp"
+        SEP_SHEAR_ANY
+#undef SEP_FMAC
     }
+#undef SEP_SHEAR_ANY
 #undef SLOT
 }
 
@@ -230,26 +244,37 @@ __device__ __forceinline__ void step_shear(double (&w)[WR], double cf)
         : "+&v"(a0), "+&v"(b0), "+&v"(a1), "+&v"(b1)                                         \
         : "v"(t0), "v"(t1), "v"(cf))
 
-template <int quad>
-__device__ __forceinline__ void gen2(double& a0, double& b0, double& a1, double& b1, double cf)
+template <typename T, int quad>
+__device__ __forceinline__ void gen2(T& a0, T& b0, T& a1, T& b1, T cf)
 {
-    const double t0 = a0, t1 = a1;
-    if constexpr (quad == 0) SEP_GEN2(0, 1, 2, 3);
-    else if constexpr (quad == 1) SEP_GEN2(4, 5, 6, 7);
-    else if constexpr (quad == 2) SEP_GEN2(8, 9, 10, 11);
+    const T t0 = a0, t1 = a1;
+#define SEP_GEN2_ANY                                                                        \
+    if constexpr (quad == 0) SEP_GEN2(0, 1, 2, 3);                                          \
+    else if constexpr (quad == 1) SEP_GEN2(4, 5, 6, 7);                                     \
+    else if constexpr (quad == 2) SEP_GEN2(8, 9, 10, 11);                                   \
     else SEP_GEN2(12, 13, 14, 15);
+    if constexpr (sizeof(T) == 8) {
+#define SEP_FMAC "v_fmac_f64_dpp"
+        SEP_GEN2_ANY
+#undef SEP_FMAC
+    } else {
+#define SEP_FMAC "v_fmac_f32_dpp"
+        SEP_GEN2_ANY
+#undef SEP_FMAC
+    }
+#undef SEP_GEN2_ANY
 }
 
-template <int KG, int s, int base, int WR>
-__device__ __forceinline__ void step_general(double (&w)[WR], double cf)
+template <typename T, int KG, int s, int base, int WR>
+__device__ __forceinline__ void step_general(T (&w)[WR], T cf)
 {
     static_assert(WR >= 2 * KG, "window ring too small");
 #define SLOT(l, o) w[(((s) - 2 * (l) + (o)) % WR + WR) % WR]
-    gen2<base / 4 + 0>(SLOT(0, 0), SLOT(0, 1), SLOT(1, 0), SLOT(1, 1), cf);
-    gen2<base / 4 + 1>(SLOT(2, 0), SLOT(2, 1), SLOT(3, 0), SLOT(3, 1), cf);
+    gen2<T, base / 4 + 0>(SLOT(0, 0), SLOT(0, 1), SLOT(1, 0), SLOT(1, 1), cf);
+    gen2<T, base / 4 + 1>(SLOT(2, 0), SLOT(2, 1), SLOT(3, 0), SLOT(3, 1), cf);
     if constexpr (KG == 8) {
-        gen2<2>(SLOT(4, 0), SLOT(4, 1), SLOT(5, 0), SLOT(5, 1), cf);
-        gen2<3>(SLOT(6, 0), SLOT(6, 1), SLOT(7, 0), SLOT(7, 1), cf);
+        gen2<T, 2>(SLOT(4, 0), SLOT(4, 1), SLOT(5, 0), SLOT(5, 1), cf);
+        gen2<T, 3>(SLOT(6, 0), SLOT(6, 1), SLOT(7, 0), SLOT(7, 1), cf);
     }
 #undef SLOT
 }
@@ -258,26 +283,37 @@ __device__ __forceinline__ void step_general(double (&w)[WR], double cf)
 // broadcast from lane s of the interval's sign row (exact: the result is x or -x).
 #define SEP_FIX(N) asm("s_nop 1\n" SEP_DPP("%0", "%1", "%0", N) : "+v"(x) : "v"(sg))
 
-template <int s>
-__device__ __forceinline__ void fix_sign(double& x, double sg)
+template <typename T, int s>
+__device__ __forceinline__ void fix_sign(T& x, T sg)
 {
     static_assert(s >= 0 && s < 16, "sign row has 16 lanes");
-    if constexpr (s == 0) SEP_FIX(0);
-    else if constexpr (s == 1) SEP_FIX(1);
-    else if constexpr (s == 2) SEP_FIX(2);
-    else if constexpr (s == 3) SEP_FIX(3);
-    else if constexpr (s == 4) SEP_FIX(4);
-    else if constexpr (s == 5) SEP_FIX(5);
-    else if constexpr (s == 6) SEP_FIX(6);
-    else if constexpr (s == 7) SEP_FIX(7);
-    else if constexpr (s == 8) SEP_FIX(8);
-    else if constexpr (s == 9) SEP_FIX(9);
-    else if constexpr (s == 10) SEP_FIX(10);
-    else if constexpr (s == 11) SEP_FIX(11);
-    else if constexpr (s == 12) SEP_FIX(12);
-    else if constexpr (s == 13) SEP_FIX(13);
-    else if constexpr (s == 14) SEP_FIX(14);
+#define SEP_FIX_ANY                                                                         \
+    if constexpr (s == 0) SEP_FIX(0);                                                       \
+    else if constexpr (s == 1) SEP_FIX(1);                                                  \
+    else if constexpr (s == 2) SEP_FIX(2);                                                  \
+    else if constexpr (s == 3) SEP_FIX(3);                                                  \
+    else if constexpr (s == 4) SEP_FIX(4);                                                  \
+    else if constexpr (s == 5) SEP_FIX(5);                                                  \
+    else if constexpr (s == 6) SEP_FIX(6);                                                  \
+    else if constexpr (s == 7) SEP_FIX(7);                                                  \
+    else if constexpr (s == 8) SEP_FIX(8);                                                  \
+    else if constexpr (s == 9) SEP_FIX(9);                                                  \
+    else if constexpr (s == 10) SEP_FIX(10);                                                \
+    else if constexpr (s == 11) SEP_FIX(11);                                                \
+    else if constexpr (s == 12) SEP_FIX(12);                                                \
+    else if constexpr (s == 13) SEP_FIX(13);                                                \
+    else if constexpr (s == 14) SEP_FIX(14);                                                \
     else SEP_FIX(15);
+    if constexpr (sizeof(T) == 8) {
+#define SEP_FMAC "v_fmac_f64_dpp"
+        SEP_FIX_ANY
+#undef SEP_FMAC
+    } else {
+#define SEP_FMAC "v_fmac_f32_dpp"
+        SEP_FIX_ANY
+#undef SEP_FMAC
+    }
+#undef SEP_FIX_ANY
 }
 
 template <int S, int s0>
@@ -307,19 +343,19 @@ struct Geom {
     static_assert(S % W == 0 && S % SPR == 0 && S <= 16, "bad interval length");
 };
 
-template <class Gm, bool FIRST, bool LAST, bool CHECKED, bool GENERAL, int D>
-__device__ __forceinline__ void interval(double* __restrict__ Ar, int64_t lda, int n,
-                                         double (&win)[Gm::W], double (&pf)[FIRST ? Gm::PFD : 1][Gm::S],
-                                         const double (&cf)[Gm::NR], double sg,
-                                         const double* lds_in, double* lds_out, int t0, int g)
+template <typename T, class Gm, bool FIRST, bool LAST, bool CHECKED, bool GENERAL, int D>
+__device__ __forceinline__ void interval(T* __restrict__ Ar, int64_t lda, int n,
+                                         T (&win)[Gm::W], T (&pf)[FIRST ? Gm::PFD : 1][Gm::S],
+                                         const T (&cf)[Gm::NR], T sg,
+                                         const T* lds_in, T* lds_out, int t0, int g)
 {
     constexpr int S = Gm::S, W = Gm::W, KG = Gm::KG, PFD = Gm::PFD;
-    double in[FIRST ? 1 : S];
+    T in[FIRST ? 1 : S];
     // first wave: prefetch slot D holds this interval's columns (loaded PFD intervals
     // ago); each is refilled, right after its step consumed it, with the column PFD
     // intervals ahead (t0 + PFD S + 1 + s)
     const int c0f = t0 + PFD * S + 1;
-    const double* __restrict__ pfp = Ar + (int64_t)c0f * lda;
+    const T* __restrict__ pfp = Ar + (int64_t)c0f * lda;
     if constexpr (!FIRST) {
         const int c0 = t0 - 2 * g * KG + 1;
 #pragma unroll
@@ -331,7 +367,7 @@ __device__ __forceinline__ void interval(double* __restrict__ Ar, int64_t lda, i
         }
     }
     const int o0 = t0 - 2 * Gm::Kp + 2;
-    double* __restrict__ po = Ar + (int64_t)o0 * lda;
+    T* __restrict__ po = Ar + (int64_t)o0 * lda;
     Steps<S, 0>::run([&](auto sc) {
         constexpr int s = decltype(sc)::value;
         constexpr int row = s / Gm::SPR, base = (s % Gm::SPR) * Gm::NC;
@@ -345,12 +381,12 @@ __device__ __forceinline__ void interval(double* __restrict__ Ar, int64_t lda, i
             win[(s + 1) % W] = in[s];
         }
         if constexpr (GENERAL)
-            step_general<KG, s, base>(win, cf[row]);
+            step_general<T, KG, s, base>(win, cf[row]);
         else
-            step_shear<KG, s, base>(win, cf[row]);
-        double out = win[(s + 2) % W];
+            step_shear<T, KG, s, base>(win, cf[row]);
+        T out = win[(s + 2) % W];
         if constexpr (LAST) {
-            if constexpr (!GENERAL) fix_sign<s>(out, sg);
+            if constexpr (!GENERAL) fix_sign<T, s>(out, sg);
             if constexpr (CHECKED) {
                 if (o0 + s >= 0 && o0 + s < n) po[(int64_t)s * lda] = out;
             } else {
@@ -400,18 +436,18 @@ __device__ __forceinline__ void block_offsets(int (&off)[CoefBlock<Gm>::PER], in
 // the output signs) from the table into registers at the start of an interval and store
 // it to LDS at the end; they issue no other global memory operations, so waiting for
 // these loads never drains the first wave's Q prefetch or the last wave's stores.
-template <class Gm>
-__device__ __forceinline__ void fetch_block(double (&v)[CoefBlock<Gm>::PER],
+template <typename T, class Gm>
+__device__ __forceinline__ void fetch_block(T (&v)[CoefBlock<Gm>::PER],
                                             const int (&off)[CoefBlock<Gm>::PER],
-                                            const double* __restrict__ tab, int bt)
+                                            const T* __restrict__ tab, int bt)
 {
-    const double* __restrict__ base = tab + (int64_t)bt * (Gm::S * Gm::NC);
+    const T* __restrict__ base = tab + (int64_t)bt * (Gm::S * Gm::NC);
 #pragma unroll
     for (int j = 0; j < CoefBlock<Gm>::PER; ++j) v[j] = base[off[j]];
 }
 
-template <class Gm>
-__device__ __forceinline__ void store_block(double* cb, const double (&v)[CoefBlock<Gm>::PER],
+template <typename T, class Gm>
+__device__ __forceinline__ void store_block(T* cb, const T (&v)[CoefBlock<Gm>::PER],
                                             int loader, int lane)
 {
     using CB = CoefBlock<Gm>;
@@ -422,12 +458,12 @@ __device__ __forceinline__ void store_block(double* cb, const double (&v)[CoefBl
     }
 }
 
-template <class Gm, bool FIRST, bool LAST, bool GENERAL>
-__device__ __forceinline__ void wave(double* __restrict__ A, int64_t lda, int m, int n,
-                                     const double* __restrict__ tab, int Up, int PADU,
+template <typename T, class Gm, bool FIRST, bool LAST, bool GENERAL>
+__device__ __forceinline__ void wave(T* __restrict__ A, int64_t lda, int m, int n,
+                                     const T* __restrict__ tab, int Up, int PADU,
                                      int T_out, int NI,
-                                     double (*ring)[2][Gm::S][kLanes],
-                                     double (*cbuf)[CoefBlock<Gm>::SIZE], int g, int lane)
+                                     T (*ring)[2][Gm::S][kLanes],
+                                     T (*cbuf)[CoefBlock<Gm>::SIZE], int g, int lane)
 {
     using CB = CoefBlock<Gm>;
     constexpr int S = Gm::S, W = Gm::W, KG = Gm::KG, PFD = Gm::PFD, NR = Gm::NR;
@@ -436,13 +472,13 @@ __device__ __forceinline__ void wave(double* __restrict__ A, int64_t lda, int m,
     // Lanes past the last row alias row m-1: they load, compute and store exactly the
     // values of that row's own lane, so no per-lane predication is needed.
     const int64_t row = min((int64_t)blockIdx.x * kLanes + lane, (int64_t)m - 1);
-    double* __restrict__ Ar = A + row;
+    T* __restrict__ Ar = A + row;
     const int cl = lane & 15;  // lane within the 16-lane DPP row
 
-    double win[W];
+    T win[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) win[w] = 0.0;
-    double pf[FIRST ? PFD : 1][S];
+    T pf[FIRST ? PFD : 1][S];
     if constexpr (FIRST) {
         // column 0 enters before step 0 (slot 0); interval d takes columns dS+1 .. dS+S
         win[0] = Ar[0];
@@ -454,12 +490,12 @@ __device__ __forceinline__ void wave(double* __restrict__ A, int64_t lda, int m,
                 pf[d][s] = (col < n) ? Ar[(int64_t)col * lda] : 0.0;
             }
     }
-    double blk[CB::PER];
+    T blk[CB::PER];
     int off[CB::PER];
     if constexpr (LOADER) {  // block of interval 0
         block_offsets<Gm>(off, Up, PADU, loader, lane);
-        fetch_block<Gm>(blk, off, tab, 0);
-        store_block<Gm>(cbuf[0], blk, loader, lane);
+        fetch_block<T, Gm>(blk, off, tab, 0);
+        store_block<T, Gm>(cbuf[0], blk, loader, lane);
     }
     lds_barrier();
     // the interval loop is unrolled by the prefetch depth so that prefetch slot D is
@@ -470,19 +506,19 @@ __device__ __forceinline__ void wave(double* __restrict__ A, int64_t lda, int m,
             const int b = b0 + D;
             if (b >= NI) return;
             if constexpr (LOADER) {
-                if (b + 1 < NI) fetch_block<Gm>(blk, off, tab, b + 1);
+                if (b + 1 < NI) fetch_block<T, Gm>(blk, off, tab, b + 1);
             }
             const int t0 = (b - g) * S + g;
             const bool active = (t0 + S > 0) && (t0 < T_out);
             if (active) {
-                const double* cb = cbuf[b & 1];
-                double cf[NR];
+                const T* cb = cbuf[b & 1];
+                T cf[NR];
 #pragma unroll
                 for (int r = 0; r < NR; ++r) cf[r] = cb[(g * NR + r) * 16 + cl];
-                const double sg = LAST ? cb[CB::ROWS + cl] : 0.0;
-                const double* lds_in =
+                const T sg = LAST ? cb[CB::ROWS + cl] : 0.0;
+                const T* lds_in =
                     FIRST ? nullptr : &ring[g > 0 ? g - 1 : 0][(b - 1) & 1][0][lane];
-                double* lds_out = LAST ? nullptr : &ring[g][b & 1][0][lane];
+                T* lds_out = LAST ? nullptr : &ring[g][b & 1][0][lane];
                 bool interior;
                 if constexpr (FIRST) {
                     interior = t0 + (PFD + 1) * S < n;
@@ -495,42 +531,42 @@ __device__ __forceinline__ void wave(double* __restrict__ A, int64_t lda, int m,
                     interior = interior && o0 >= 0 && o0 + S <= n;
                 }
                 if (interior)
-                    interval<Gm, FIRST, LAST, false, GENERAL, D>(Ar, lda, n, win, pf, cf, sg,
+                    interval<T, Gm, FIRST, LAST, false, GENERAL, D>(Ar, lda, n, win, pf, cf, sg,
                                                                  lds_in, lds_out, t0, g);
                 else
-                    interval<Gm, FIRST, LAST, true, GENERAL, D>(Ar, lda, n, win, pf, cf, sg,
+                    interval<T, Gm, FIRST, LAST, true, GENERAL, D>(Ar, lda, n, win, pf, cf, sg,
                                                                 lds_in, lds_out, t0, g);
             }
             if constexpr (LOADER) {
-                if (b + 1 < NI) store_block<Gm>(cbuf[(b + 1) & 1], blk, loader, lane);
+                if (b + 1 < NI) store_block<T, Gm>(cbuf[(b + 1) & 1], blk, loader, lane);
             }
             lds_barrier();
         });
     }
 }
 
-template <class Gm>
+template <typename T, class Gm>
 __global__ void __launch_bounds__(kLanes * Gm::G)
-    k_apply_dpp(double* __restrict__ A, int64_t lda, int m, int n,
-                const double* __restrict__ tab, const int* __restrict__ flag, int Up, int PADU,
+    k_apply_dpp(T* __restrict__ A, int64_t lda, int m, int n,
+                const T* __restrict__ tab, const int* __restrict__ flag, int Up, int PADU,
                 int T_out, int NI)
 {
     constexpr int G = Gm::G;
     static_assert(G >= 3, "the coefficient loaders are the middle waves");
-    __shared__ double ring[G - 1][2][Gm::S][kLanes];
-    __shared__ double cbuf[2][CoefBlock<Gm>::SIZE];
+    __shared__ T ring[G - 1][2][Gm::S][kLanes];
+    __shared__ T cbuf[2][CoefBlock<Gm>::SIZE];
     const int g = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
     const int lane = threadIdx.x % kLanes;
     const bool general = *flag != 0;
 #define SEP_WAVE(GEN)                                                                         \
     if (g == 0)                                                                               \
-        wave<Gm, true, false, GEN>(A, lda, m, n, tab, Up, PADU, T_out, NI, ring, cbuf, g,     \
+        wave<T, Gm, true, false, GEN>(A, lda, m, n, tab, Up, PADU, T_out, NI, ring, cbuf, g,     \
                                    lane);                                                     \
     else if (g == G - 1)                                                                      \
-        wave<Gm, false, true, GEN>(A, lda, m, n, tab, Up, PADU, T_out, NI, ring, cbuf, g,     \
+        wave<T, Gm, false, true, GEN>(A, lda, m, n, tab, Up, PADU, T_out, NI, ring, cbuf, g,     \
                                    lane);                                                     \
     else                                                                                      \
-        wave<Gm, false, false, GEN>(A, lda, m, n, tab, Up, PADU, T_out, NI, ring, cbuf, g,    \
+        wave<T, Gm, false, false, GEN>(A, lda, m, n, tab, Up, PADU, T_out, NI, ring, cbuf, g,    \
                                     lane);
     if (general) {
         SEP_WAVE(true)
@@ -540,46 +576,46 @@ __global__ void __launch_bounds__(kLanes * Gm::G)
 #undef SEP_WAVE
 }
 
-template <class Gm>
-int run_dpp(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, double* A,
-            int64_t lda, const double* cs, int64_t ld_cs, int64_t k, bool rotations)
+template <typename T, class Gm>
+int run_dpp(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T* A,
+            int64_t lda, const T* cs, int64_t ld_cs, int64_t k, bool rotations)
 {
     constexpr int G = Gm::G, S = Gm::S, KG = Gm::KG, Kp = Gm::Kp;
     const int T_out = (int)(n + 2 * Kp - 2);
     const int NI = G + (T_out + S - 1) / S + 1;
     const int PADU = (G - 1) * (S - 1);
     const int Up = NI * S + 2 * S + PADU;
-    if (int rc = w.table.reserve(sizeof(double) * ((size_t)(G + 1) * Up * Gm::NC + 64))) return rc;
-    if (int rc = w.aux.reserve(sizeof(uint64_t) * (size_t)n + sizeof(double) * (size_t)n + 256))
+    if (int rc = w.table.reserve(sizeof(T) * ((size_t)(G + 1) * Up * Gm::NC + 64))) return rc;
+    if (int rc = w.aux.reserve(sizeof(uint64_t) * (size_t)n + sizeof(T) * (size_t)n + 256))
         return rc;
     uint64_t* mask = w.aux.as<uint64_t>();
-    double* sigm1 = reinterpret_cast<double*>(mask + n);
+    T* sigm1 = reinterpret_cast<T*>(mask + n);
     int* flag = reinterpret_cast<int*>(sigm1 + n);
     const int tp0 = timing_mark(h, st);
     SEP_HIP(hipMemsetAsync(flag, 0, sizeof(int), st));
     if (!rotations) {
         const int64_t tot = (n - 1) * k;
         const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 2048);
-        hipLaunchKernelGGL(k_check_norm, dim3(blocks), dim3(256), 0, st, cs, ld_cs, (int)n,
+        hipLaunchKernelGGL(k_check_norm<T>, dim3(blocks), dim3(256), 0, st, cs, ld_cs, (int)n,
                            (int)k, flag);
         SEP_CHECK_LAUNCH();
     }
     const int64_t rb = (m + kLanes - 1) / kLanes;
     int tp_prev = tp0;
     for (int64_t L0 = 0; L0 < k; L0 += Kp) {
-        hipLaunchKernelGGL(k_colsigns, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cs,
+        hipLaunchKernelGGL(k_colsigns<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cs,
                            ld_cs, (int)n, (int)k, (int)L0, Kp, flag, mask, sigm1);
         SEP_CHECK_LAUNCH();
         const int64_t total = (int64_t)G * Up * KG + (int64_t)NI * 16;
         const int tblocks = (int)std::min<int64_t>((total + 255) / 256, 8192);
-        hipLaunchKernelGGL((k_coef_table<KG, G, S>), dim3(tblocks), dim3(256), 0, st,
-                           w.table.as<double>(), cs, ld_cs, (int)n, (int)k, (int)L0, Up, PADU,
+        hipLaunchKernelGGL((k_coef_table<T, KG, G, S>), dim3(tblocks), dim3(256), 0, st,
+                           w.table.as<T>(), cs, ld_cs, (int)n, (int)k, (int)L0, Up, PADU,
                            NI, flag, mask, sigm1);
         SEP_CHECK_LAUNCH();
         const int ta = timing_mark(h, st);
         timing_pair(h, false, tp_prev, ta);
-        hipLaunchKernelGGL((k_apply_dpp<Gm>), dim3((unsigned)rb), dim3(kLanes * G), 0, st, A,
-                           lda, (int)m, (int)n, w.table.as<double>(), flag, Up, PADU, T_out, NI);
+        hipLaunchKernelGGL((k_apply_dpp<T, Gm>), dim3((unsigned)rb), dim3(kLanes * G), 0, st, A,
+                           lda, (int)m, (int)n, w.table.as<T>(), flag, Up, PADU, T_out, NI);
         SEP_CHECK_LAUNCH();
         tp_prev = timing_mark(h, st);
         timing_pair(h, true, ta, tp_prev);
@@ -632,12 +668,13 @@ struct G16 {
 };
 
 // Per-wave constants of the 16-row kernel.
+template <typename T>
 struct R16Args {
-    double* A0;            // A + first row of the block (uniform)
+    T* A0;            // A + first row of the block (uniform)
     int64_t voff;          // lane offset: (lane / RPG) lda + row within the block
     int64_t lda;
-    const double* tab;
-    const double* sigm1;
+    const T* tab;
+    const T* sigm1;
     int n, NBt, w, lane;
     bool first, last;
 };
@@ -645,9 +682,9 @@ struct R16Args {
 // tab[((w NBt + b) NR + rr) 64 + lane]: the coefficient register rr of interval b of wave
 // w.  Lane 16 r + j of it belongs to group q = 4 w + r, step u = b S + rr SPR + j / NC,
 // layer q KG + (j % NC) / 2; even j hold p (shear) or c - 1 (general), odd j hold q or s.
-template <class Gm>
-__global__ void __launch_bounds__(256) k_coef_r16(double* __restrict__ tab,
-                                                  const double* __restrict__ cs, int64_t ld,
+template <typename T, class Gm>
+__global__ void __launch_bounds__(256) k_coef_r16(T* __restrict__ tab,
+                                                  const T* __restrict__ cs, int64_t ld,
                                                   int n, int k, int L0, int NBt,
                                                   const int* __restrict__ flag,
                                                   const uint64_t* __restrict__ mask)
@@ -669,17 +706,17 @@ __global__ void __launch_bounds__(256) k_coef_r16(double* __restrict__ tab,
         const int lr = q * KG + (j % NC) / 2;
         const int L = L0 + lr;
         const int64_t i = u - (int64_t)q * (S - 1) - 2 * lr;
-        double v = 0.0;
+        T v = 0.0;
         if (L < k && i >= 0 && i <= n - 2) {
-            const double c = cs[2 * (i + ld * L)], s = cs[2 * (i + ld * L) + 1];
+            const T c = cs[2 * (i + ld * L)], s = cs[2 * (i + ld * L) + 1];
             if (general) {
                 v = (j & 1) ? s : c - 1.0;
             } else {
                 // stored signs of columns i and i+1 when rotation (i, L) is applied
                 bool na = (mask[i] >> lr) & 1;
-                na ^= neg_c(cs, ld, n, i - 1, L, k);
+                na ^= neg_c<T>(cs, ld, n, i - 1, L, k);
                 const bool nb = (mask[i + 1] >> lr) & 1;
-                const double st = ((c < 0.0) ^ na ^ nb) ? -s : s;
+                const T st = ((c < 0.0) ^ na ^ nb) ? -s : s;
                 v = (j & 1) ? -st : st / (1.0 + fabs(c));
             }
         }
@@ -690,11 +727,11 @@ __global__ void __launch_bounds__(256) k_coef_r16(double* __restrict__ tab,
 // One interval (S steps) of every wave.  IU is the interval's index in the unrolled loop
 // body: its parity P selects the LDS buffers and its first local step fixes the window
 // slots, which are compile-time registers (the same for every lane).
-template <class Gm, bool GENERAL, int IU>
-__device__ __forceinline__ void r16_interval(const R16Args& a, double (*buf)[2][Gm::S][kLanes],
-                                             double (&win)[Gm::WR], double (&cf)[2][Gm::NR],
-                                             double (&qv)[2][Gm::NQ], const double* rd,
-                                             double* wr, int b)
+template <typename T, class Gm, bool GENERAL, int IU>
+__device__ __forceinline__ void r16_interval(const R16Args<T>& a, T (*buf)[2][Gm::S][kLanes],
+                                             T (&win)[Gm::WR], T (&cf)[2][Gm::NR],
+                                             T (&qv)[2][Gm::NQ], const T* rd,
+                                             T* wr, int b)
 {
     constexpr int S = Gm::S, WR = Gm::WR, KG = Gm::KG, NW = Gm::NW, NR = Gm::NR;
     constexpr int NQ = Gm::NQ, SPR = Gm::SPR, NC = Gm::NC, W = Gm::W;
@@ -706,7 +743,7 @@ __device__ __forceinline__ void r16_interval(const R16Args& a, double (*buf)[2][
 #pragma unroll
     for (int s = 0; s < S; ++s) win[(s0 + s + 1) % WR] = rd[((1 - P) * S + s) * kLanes];
     // last wave: its outputs of interval b - 1, columns (b - 1) S + 4 k + r - DL
-    double ob[NQ], sg[NQ];
+    T ob[NQ], sg[NQ];
     const int oc0 = (b - 1) * S - Gm::DL;  // output column of lane row 0, k = 0
     const bool oful = oc0 >= 0 && oc0 + S <= a.n;
     if (a.last) {
@@ -722,14 +759,14 @@ __device__ __forceinline__ void r16_interval(const R16Args& a, double (*buf)[2][
         constexpr int sw = (s0 + s) % WR;
         constexpr int rr = s / SPR, base = (s % SPR) * NC;
         if constexpr (GENERAL)
-            step_general<KG, sw, base>(win, cf[P][rr]);
+            step_general<T, KG, sw, base>(win, cf[P][rr]);
         else
-            step_shear<KG, sw, base>(win, cf[P][rr]);
+            step_shear<T, KG, sw, base>(win, cf[P][rr]);
         // the column leaving the window entered W - 1 steps ago
         wr[(P * S + s) * kLanes] = win[(sw + 2 - W + WR) % WR];
     });
     // coefficients of interval b + 2 (the table has NB + 2 intervals)
-    const double* __restrict__ tb = a.tab + ((int64_t)(a.w * a.NBt + b + 2) * NR) * kLanes;
+    const T* __restrict__ tb = a.tab + ((int64_t)(a.w * a.NBt + b + 2) * NR) * kLanes;
 #pragma unroll
     for (int rr = 0; rr < NR; ++rr) cf[P][rr] = tb[rr * kLanes + lane];
     if (a.first) {
@@ -738,7 +775,7 @@ __device__ __forceinline__ void r16_interval(const R16Args& a, double (*buf)[2][
 #pragma unroll
         for (int k4 = 0; k4 < NQ; ++k4) buf[0][P][GPW * k4 + r][OL + j] = qv[P][k4];
         const int c0 = (b + 3) * S + 1;
-        const double* __restrict__ src = a.A0 + (int64_t)c0 * a.lda;
+        const T* __restrict__ src = a.A0 + (int64_t)c0 * a.lda;
         if (c0 + S <= a.n) {
 #pragma unroll
             for (int k4 = 0; k4 < NQ; ++k4) qv[P][k4] = src[(int64_t)GPW * k4 * a.lda + a.voff];
@@ -749,7 +786,7 @@ __device__ __forceinline__ void r16_interval(const R16Args& a, double (*buf)[2][
         }
     }
     if (a.last) {
-        double* __restrict__ dst = a.A0 + (int64_t)oc0 * a.lda;
+        T* __restrict__ dst = a.A0 + (int64_t)oc0 * a.lda;
         if (oful) {
 #pragma unroll
             for (int k4 = 0; k4 < NQ; ++k4)
@@ -766,14 +803,14 @@ __device__ __forceinline__ void r16_interval(const R16Args& a, double (*buf)[2][
     lds_barrier();
 }
 
-template <class Gm, bool GENERAL>
-__device__ __forceinline__ void r16_body(double* __restrict__ A, int64_t lda, int m, int n,
-                                         const double* __restrict__ tab,
-                                         const double* __restrict__ sigm1, int NB, int NBt,
-                                         double (*buf)[2][Gm::S][kLanes])
+template <typename T, class Gm, bool GENERAL>
+__device__ __forceinline__ void r16_body(T* __restrict__ A, int64_t lda, int m, int n,
+                                         const T* __restrict__ tab,
+                                         const T* __restrict__ sigm1, int NB, int NBt,
+                                         T (*buf)[2][Gm::S][kLanes])
 {
     constexpr int S = Gm::S, WR = Gm::WR, NW = Gm::NW, NR = Gm::NR, NQ = Gm::NQ, U = Gm::U;
-    R16Args a;
+    R16Args<T> a;
     a.w = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
     a.lane = threadIdx.x % kLanes;
     a.first = a.w == 0;
@@ -790,28 +827,28 @@ __device__ __forceinline__ void r16_body(double* __restrict__ A, int64_t lda, in
     const int jr = (int)(min(row0 + j, (int64_t)m - 1) - row0);
     a.A0 = A + row0;                          // uniform
     a.voff = (int64_t)r * lda + jr;           // lane RPG c + j: column c, row j
-    const double* rd = r == 0 ? &buf[a.w][0][0][OL + j] : &buf[a.w + 1][0][0][a.lane - RPG];
-    double* wr = &buf[a.w + 1][0][0][a.lane];
+    const T* rd = r == 0 ? &buf[a.w][0][0][OL + j] : &buf[a.w + 1][0][0][a.lane - RPG];
+    T* wr = &buf[a.w + 1][0][0][a.lane];
 
-    double win[WR];
+    T win[WR];
 #pragma unroll
     for (int s = 0; s < WR; ++s) win[s] = 0.0;
     // interval 0 reads parity 1: zero every wave's buffer, stage columns 1 .. S
 #pragma unroll
     for (int s = 0; s < S; ++s) wr[(S + s) * kLanes] = 0.0;
-    double qv[2][NQ];
+    T qv[2][NQ];
     if (a.first) {
         if (r == 0) win[0] = A[row0 + jr];  // column 0 sits in slot 0 before step 0
 #pragma unroll
         for (int k4 = 0; k4 < NQ; ++k4) {
             const int c0 = 1 + GPW * k4 + r, c1 = S + c0, c2 = 2 * S + c0;
-            const double* __restrict__ p = a.A0 + (int64_t)(GPW * k4) * lda + a.voff;
+            const T* __restrict__ p = a.A0 + (int64_t)(GPW * k4) * lda + a.voff;
             buf[0][1][GPW * k4 + r][OL + j] = c0 < n ? p[(int64_t)1 * lda] : 0.0;
             qv[0][k4] = c1 < n ? p[(int64_t)(S + 1) * lda] : 0.0;
             qv[1][k4] = c2 < n ? p[(int64_t)(2 * S + 1) * lda] : 0.0;
         }
     }
-    double cf[2][NR];
+    T cf[2][NR];
 #pragma unroll
     for (int rr = 0; rr < NR; ++rr) {
         cf[0][rr] = tab[((int64_t)(a.w * NBt + 0) * NR + rr) * kLanes + a.lane];
@@ -821,7 +858,7 @@ __device__ __forceinline__ void r16_body(double* __restrict__ A, int64_t lda, in
     for (int b = 0; b < NB; b += U) {
         Steps<U, 0>::run([&](auto ic) {
             constexpr int IU = decltype(ic)::value;
-            r16_interval<Gm, GENERAL, IU>(a, buf, win, cf, qv, rd, wr, b + IU);
+            r16_interval<T, Gm, GENERAL, IU>(a, buf, win, cf, qv, rd, wr, b + IU);
         });
     }
     // outputs of the last interval (NB - 1, parity 1)
@@ -830,66 +867,66 @@ __device__ __forceinline__ void r16_body(double* __restrict__ A, int64_t lda, in
         for (int k4 = 0; k4 < NQ; ++k4) {
             const int oc = (NB - 1) * S + GPW * k4 + r - Gm::DL;
             if (oc >= 0 && oc < n) {
-                const double x = buf[NW][1][GPW * k4 + r][OL + j];
+                const T x = buf[NW][1][GPW * k4 + r][OL + j];
                 a.A0[(int64_t)(oc - r) * lda + a.voff] = fma(sigm1[oc], x, x);
             }
         }
     }
 }
 
-template <class Gm>
+template <typename T, class Gm>
 __global__ void __launch_bounds__(kLanes * Gm::NW)
-    k_apply_r16(double* __restrict__ A, int64_t lda, int m, int n,
-                const double* __restrict__ tab, const int* __restrict__ flag,
-                const double* __restrict__ sigm1, int NB, int NBt)
+    k_apply_r16(T* __restrict__ A, int64_t lda, int m, int n,
+                const T* __restrict__ tab, const int* __restrict__ flag,
+                const T* __restrict__ sigm1, int NB, int NBt)
 {
-    __shared__ double buf[Gm::NW + 1][2][Gm::S][kLanes];
+    __shared__ T buf[Gm::NW + 1][2][Gm::S][kLanes];
     if (*flag != 0)
-        r16_body<Gm, true>(A, lda, m, n, tab, sigm1, NB, NBt, buf);
+        r16_body<T, Gm, true>(A, lda, m, n, tab, sigm1, NB, NBt, buf);
     else
-        r16_body<Gm, false>(A, lda, m, n, tab, sigm1, NB, NBt, buf);
+        r16_body<T, Gm, false>(A, lda, m, n, tab, sigm1, NB, NBt, buf);
 }
 // (R16_END)
 
-template <class Gm>
-int run_r16(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, double* A,
-            int64_t lda, const double* cs, int64_t ld_cs, int64_t k, bool rotations)
+template <typename T, class Gm>
+int run_r16(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T* A,
+            int64_t lda, const T* cs, int64_t ld_cs, int64_t k, bool rotations)
 {
     constexpr int S = Gm::S, Kp = Gm::Kp;
     int NB = (int)((n + Gm::DL + S - 1) / S);
     NB = (NB + Gm::U - 1) / Gm::U * Gm::U;  // whole unrolled loop bodies
     const int NBt = NB + 2;
     const size_t tab_elems = (size_t)Gm::NW * NBt * Gm::NR * kLanes;
-    if (int rc = w.table.reserve(sizeof(double) * tab_elems)) return rc;
-    if (int rc = w.aux.reserve(sizeof(uint64_t) * (size_t)n + sizeof(double) * (size_t)n + 256))
+    if (int rc = w.table.reserve(sizeof(T) * tab_elems)) return rc;
+    if (int rc = w.aux.reserve(sizeof(uint64_t) * (size_t)n + sizeof(T) * (size_t)n + 256))
         return rc;
     uint64_t* mask = w.aux.as<uint64_t>();
-    double* sigm1 = reinterpret_cast<double*>(mask + n);
+    T* sigm1 = reinterpret_cast<T*>(mask + n);
     int* flag = reinterpret_cast<int*>(sigm1 + n);
     const int tp0 = timing_mark(h, st);
     SEP_HIP(hipMemsetAsync(flag, 0, sizeof(int), st));
     if (!rotations) {
         const int64_t tot = (n - 1) * k;
         const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 2048);
-        hipLaunchKernelGGL(k_check_norm, dim3(blocks), dim3(256), 0, st, cs, ld_cs, (int)n,
+        hipLaunchKernelGGL(k_check_norm<T>, dim3(blocks), dim3(256), 0, st, cs, ld_cs, (int)n,
                            (int)k, flag);
         SEP_CHECK_LAUNCH();
     }
     const int64_t wgs = (m + Gm::RPG - 1) / Gm::RPG;
     int tp_prev = tp0;
     for (int64_t L0 = 0; L0 < k; L0 += Kp) {
-        hipLaunchKernelGGL(k_colsigns, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cs,
+        hipLaunchKernelGGL(k_colsigns<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cs,
                            ld_cs, (int)n, (int)k, (int)L0, Kp, flag, mask, sigm1);
         SEP_CHECK_LAUNCH();
         const int tblocks = (int)std::min<int64_t>(((int64_t)tab_elems + 255) / 256, 8192);
-        hipLaunchKernelGGL((k_coef_r16<Gm>), dim3(tblocks), dim3(256), 0, st,
-                           w.table.as<double>(), cs, ld_cs, (int)n, (int)k, (int)L0, NBt, flag,
+        hipLaunchKernelGGL((k_coef_r16<T, Gm>), dim3(tblocks), dim3(256), 0, st,
+                           w.table.as<T>(), cs, ld_cs, (int)n, (int)k, (int)L0, NBt, flag,
                            mask);
         SEP_CHECK_LAUNCH();
         const int ta = timing_mark(h, st);
         timing_pair(h, false, tp_prev, ta);
-        hipLaunchKernelGGL((k_apply_r16<Gm>), dim3((unsigned)wgs), dim3(kLanes * Gm::NW), 0, st,
-                           A, lda, (int)m, (int)n, w.table.as<double>(), flag, sigm1, NB, NBt);
+        hipLaunchKernelGGL((k_apply_r16<T, Gm>), dim3((unsigned)wgs), dim3(kLanes * Gm::NW), 0, st,
+                           A, lda, (int)m, (int)n, w.table.as<T>(), flag, sigm1, NB, NBt);
         SEP_CHECK_LAUNCH();
         tp_prev = timing_mark(h, st);
         timing_pair(h, true, ta, tp_prev);
@@ -905,9 +942,9 @@ using G32Deep = G16<4, 8, 8, 32>;   // 32-row blocks: 8 waves of 2 groups of 4 l
 
 }  // namespace
 
-int bulk_apply_dpp_f64(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n,
-                       double* A, int64_t lda, const double* cs, int64_t ld_cs, int64_t k,
-                       bool rotations)
+template <typename T>
+int bulk_apply_dpp(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T* A,
+                   int64_t lda, const T* cs, int64_t ld_cs, int64_t k, bool rotations)
 {
     // Measured kernel time per apply, m = n, k = 64 (ms):
     //            wide   deep   r16    r16deep r32    r32deep
@@ -924,19 +961,32 @@ int bulk_apply_dpp_f64(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, i
     // waves); neighbour-to-neighbour interval counters in LDS instead of the workgroup
     // barrier 2.66 ms.
     const int geom = h ? h->apply_geom : 0;
-    if (geom == 2) return run_dpp<GeomWide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if (geom == 1) return run_dpp<GeomDeep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if (geom == 5) return run_dpp<GeomSmall>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if (geom == 6) return run_r16<G16Wide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if (geom == 7) return run_r16<G16Deep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if (geom == 8) return run_r16<G16Small>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if (geom == 9) return run_r16<G32Wide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if (geom == 10) return run_r16<G32Deep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if (k <= 32) return run_dpp<GeomSmall>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (geom == 2) return run_dpp<T, GeomWide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (geom == 1) return run_dpp<T, GeomDeep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (geom == 5) return run_dpp<T, GeomSmall>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (geom == 6) return run_r16<T, G16Wide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (geom == 7) return run_r16<T, G16Deep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (geom == 8) return run_r16<T, G16Small>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (geom == 9) return run_r16<T, G32Wide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (geom == 10) return run_r16<T, G32Deep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (k <= 32) return run_dpp<T, GeomSmall>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if constexpr (sizeof(T) == 4) {
+        // Float32, m = n, k = 64 (ms): 2000: wide 0.276, r16deep 0.164; 4096: 0.531, 0.309;
+        // 8192: 1.030, 0.700; 16384: 2.038, 2.191 (r32 0.22 / 0.42 / 0.88 / 2.15).  The
+        // scalar-fed kernel of apply.hip (SEP_APPLY=sgpr) takes 0.721 / 1.361 / 2.649 / 5.849.
+        if (m <= 12288) return run_r16<T, G16Deep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+        return run_dpp<T, GeomWide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    }
     if (m <= 6144 || (m > 10240 && m <= 14336))
-        return run_r16<G16Deep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if (m <= 10240) return run_r16<G32Wide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    return run_dpp<GeomWide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+        return run_r16<T, G16Deep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (m <= 10240) return run_r16<T, G32Wide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    return run_dpp<T, GeomWide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
 }
+
+template int bulk_apply_dpp<double>(sep_handle*, hipStream_t, ApplyWork&, int64_t, int64_t,
+                                    double*, int64_t, const double*, int64_t, int64_t, bool);
+template int bulk_apply_dpp<float>(sep_handle*, hipStream_t, ApplyWork&, int64_t, int64_t,
+                                   float*, int64_t, const float*, int64_t, int64_t, bool);
+
 
 }  // namespace sep

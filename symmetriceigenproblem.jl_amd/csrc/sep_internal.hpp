@@ -97,7 +97,7 @@ struct sep_handle {
     int tune_choice = -1;
     int force_kf = 0, force_r = 0;  // sep_set_batched_config (0: autotuned / default)
     bool force_sgpr_apply = false;  // SEP_APPLY=sgpr selects the scalar-fed Float64 kernel
-    int apply_geom = 0;  // Float64 apply geometry forced by SEP_APPLY (0: chosen by size)
+    int apply_geom = 0;  // apply geometry forced by SEP_APPLY (0: chosen by size)
     // kernel timing (sep_set_timing): event pairs recorded around apply / prep kernels
     bool timing = false;
     std::vector<hipEvent_t> tev;            // pool, reused
@@ -120,10 +120,10 @@ namespace sep {
 template <typename T>
 int bulk_apply_dev(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T* A,
                    int64_t lda, const T* cs, int64_t ld_cs, int64_t k, bool rotations = false);
-// apply_dpp.hip (Float64 DPP-fed kernels)
-int bulk_apply_dpp_f64(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n,
-                       double* A, int64_t lda, const double* cs, int64_t ld_cs, int64_t k,
-                       bool rotations);
+// apply_dpp.hip (DPP-fed kernels; T = double or float)
+template <typename T>
+int bulk_apply_dpp(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T* A,
+                   int64_t lda, const T* cs, int64_t ld_cs, int64_t k, bool rotations);
 
 // chase.hip
 template <typename T>
