@@ -14,15 +14,26 @@
 //    row, loaded with ordinary vector loads one whole interval ahead, and every FMA takes
 //    its coefficient through the DPP64 `row_newbcast:j` source modifier, which broadcasts
 //    lane j of each row.
-//  * The row_newbcast operand costs issue rate: straight-line FP64 FMAs with a plain VGPR
-//    operand issue at ~96% of peak from one wave per SIMD, with the DPP operand at 67%
-//    (one wave per SIMD), 80% (two), 87% (four) and 94% (eight).  Two 64-row geometries
-//    are built: "wide" (G = 8 waves of KG = 8 layers, S = 16 steps per barrier interval,
-//    two waves per SIMD) and "deep" (G = 16 waves of KG = 4 layers, S = 8, four waves per
-//    SIMD); wide is used for large matrices because deep's doubled barrier count costs
-//    more than its occupancy gains.  The first wave prefetches Q two or three intervals
-//    ahead.  Below 64 * 256 rows these leave CUs idle; the 16- and 32-row geometries
-//    further down (one layer group per DPP row, several workgroups per CU) take over.
+//  * FP64 FMA issue rate depends on how many waves share a SIMD, not on the DPP operand:
+//    straight-line FMA streams (plain VGPR, SGPR or row_newbcast operand alike) reach
+//    ~66% of peak with one wave per SIMD, ~79% with two, ~87% with four and ~92% with
+//    eight; this kernel's three-shear pattern 75% / 86% with two / four.  Two 64-row
+//    geometries are built: "wide" (G = 8 waves of KG = 8 layers, S = 16 steps per barrier
+//    interval, two waves per SIMD) and "deep" (G = 16 waves of KG = 4 layers, S = 8, four
+//    waves per SIMD); wide is used for large matrices: deep synchronises twice as often
+//    and moves twice the LDS traffic per FMA, and the chip runs it at a lower clock (the
+//    apply is power-limited: ~1.85 GHz for deep against ~2.0 GHz for wide).  The first
+//    wave prefetches Q two or three intervals ahead.  Below 64 * 256 rows these leave CUs
+//    idle; the 16- and 32-row geometries further down (one layer group per DPP row,
+//    several workgroups per CU) take over.
+//  * Within an interval each wave lowers its priority as it progresses, so the younger of
+//    two waves sharing a SIMD is not starved until the older one waits at the barrier,
+//    and reads its LDS inputs and coefficient rows two steps ahead instead of all at the
+//    interval start (where the whole workgroup's reads queued behind each other).
+//  * Q is loaded and stored through one buffer resource per column (built with scalar
+//    instructions): the hardware range check returns 0 for columns past n and drops
+//    stores outside the matrix, so every wave runs every interval on one code path with
+//    no per-lane address arithmetic.
 //  * Only the middle waves load coefficients (for every wave, one interval ahead, staged
 //    through LDS); the first wave issues only Q loads and the last only Q stores, so no
 //    wave ever has to wait for a memory operation it issued for a later interval.
@@ -31,9 +42,11 @@
 //    p = s~/(1 + c~) = tan(phi/2), q = -s~: three in-place FMAs per rotation and row instead
 //    of two multiplies and two FMAs.  To keep |p| <= 1 the rotation is taken with c~ = |c|;
 //    the sign this drops is tracked per column (each rotation with c < 0 negates both of
-//    its columns in the stored representation), folded into s~, and removed when a column
-//    leaves the pass.  The column signs are prefix products over the layers, computed by a
-//    small per-column kernel.
+//    its columns in the stored representation) and folded into s~.  The systolic kernel
+//    multiplies each column by its final sign when it enters the pass (first wave), so it
+//    leaves with a positive stored sign; the 16- and 32-row kernels remove the sign when
+//    the column leaves.  The column signs are prefix products over the layers, computed
+//    by a small per-column kernel.
 //  * Grids that are not rotations (the reference benchmark's (rand, rand) pairs) take a
 //    five-instruction path, t = a; a += (c-1) a; a += s b; b += (c-1) b; b -= s t, chosen
 //    on the device by a normalisation check, so the entry point accepts any (c, s) as the
@@ -72,10 +85,13 @@ __device__ __forceinline__ bool neg_c(const T* __restrict__ cs, int64_t ld, int 
 }
 
 // Per column j: bit lr of mask[j] = the stored sign of column j is negative before the
-// rotations of layer L0 + lr; sigm1[j] = (final sign) - 1, i.e. 0 or -2.
+// rotations of layer L0 + lr; sigm1[j] = (final sign) - 1, i.e. 0 or -2.  With in_fix the
+// column is stored with its final sign from the start (the kernel multiplies it by that
+// sign when it enters the pass), so the bits are taken relative to it and the column
+// leaves the pass with a positive stored sign.
 template <typename T>
 __global__ void __launch_bounds__(256) k_colsigns(const T* __restrict__ cs, int64_t ld,
-                                                  int n, int k, int L0, int Kp,
+                                                  int n, int k, int L0, int Kp, int in_fix,
                                                   const int* __restrict__ flag,
                                                   uint64_t* __restrict__ mask,
                                                   T* __restrict__ sigm1)
@@ -96,6 +112,7 @@ __global__ void __launch_bounds__(256) k_colsigns(const T* __restrict__ cs, int6
         // rotations (j-1, L) then (j, L) touch column j in layer L
         neg ^= neg_c<T>(cs, ld, n, j - 1, L, k) ^ neg_c<T>(cs, ld, n, j, L, k);
     }
+    if (in_fix && neg) m = ~m;
     mask[j] = m;
     sigm1[j] = neg ? -2.0 : 0.0;
 }
@@ -105,9 +122,10 @@ __global__ void __launch_bounds__(256) k_colsigns(const T* __restrict__ cs, int6
 // inside the table).  Slot l of wave g at time t is rotation i = t - 2 (g KG + l) of layer
 // L0 + g KG + l; [2l] = p, [2l+1] = q (shear form) or c - 1, s (general form);
 // identity / out-of-grid slots hold (0, 0), which is the identity in both forms.
-// Section g = G holds, for every interval bt, the last wave's 16-lane output-sign row at
-// the position a coefficient row of a (virtual) wave G would have, so the coefficient
-// loaders fetch it with the same address pattern.
+// Section g = G holds, for every interval bt, the first wave's 16-lane input-sign row
+// (sigm1 of the columns bt S + 1 .. bt S + S that enter the pass in that interval) at the
+// position a coefficient row of a (virtual) wave G would have, so the coefficient loaders
+// fetch it with the same address pattern.
 template <typename T, int KG, int G, int S>
 __global__ void __launch_bounds__(256) k_coef_table(T* __restrict__ tab,
                                                     const T* __restrict__ cs, int64_t ld,
@@ -116,7 +134,7 @@ __global__ void __launch_bounds__(256) k_coef_table(T* __restrict__ tab,
                                                     const uint64_t* __restrict__ mask,
                                                     const T* __restrict__ sigm1)
 {
-    constexpr int NC = 2 * KG, Kp = KG * G;
+    constexpr int NC = 2 * KG;
     const int64_t rot_slots = (int64_t)G * Up * KG;
     const int64_t total = rot_slots + (int64_t)NI * 16;
     const int general = *flag;
@@ -125,8 +143,7 @@ __global__ void __launch_bounds__(256) k_coef_table(T* __restrict__ tab,
         if (q >= rot_slots) {  // output-sign section
             const int qq = (int)(q - rot_slots);
             const int bt = qq / 16, c = qq % 16;
-            const int tl = (bt - (G - 1)) * S + (G - 1);
-            const int col = tl - 2 * Kp + 2 + c;
+            const int col = bt * S + 1 + c;
             const T v = (c < S && col >= 0 && col < n && !general) ? sigm1[col] : 0.0;
             tab[(int64_t)G * Up * NC + (int64_t)(bt * S + G * (1 - S) + S + PADU) * NC + c] = v;
             continue;
@@ -343,40 +360,87 @@ struct Geom {
     static_assert(S % W == 0 && S % SPR == 0 && S <= 16, "bad interval length");
 };
 
-template <typename T, class Gm, bool FIRST, bool LAST, bool CHECKED, bool GENERAL, int D>
-__device__ __forceinline__ void interval(T* __restrict__ Ar, int64_t lda, int n,
-                                         T (&win)[Gm::W], T (&pf)[FIRST ? Gm::PFD : 1][Gm::S],
-                                         const T (&cf)[Gm::NR], T sg,
-                                         const T* lds_in, T* lds_out, int t0, int g)
+// One column of the 64-row tile at At (uniform) through a buffer resource covering exactly
+// that column when it lies in [0, n) and nothing otherwise: the hardware range check makes
+// loads of columns outside the matrix return 0 and drops stores to them, so neither needs
+// a branch or a per-lane address (voff = the lane's row offset in bytes).
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t col_rsrc(T* p, int col, int n)
 {
-    constexpr int S = Gm::S, W = Gm::W, KG = Gm::KG, PFD = Gm::PFD;
+    const int bytes = (col >= 0 && col < n) ? kLanes * (int)sizeof(T) : 0;
+    return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, bytes, 0x00020000);
+}
+// p = the column's first element in this tile (uniform), col = its index
+template <typename T>
+__device__ __forceinline__ T col_load(T* p, int col, int n, uint32_t voff)
+{
+    const __amdgpu_buffer_rsrc_t r = col_rsrc(p, col, n);
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
+    else
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0));
+}
+template <typename T>
+__device__ __forceinline__ void col_store(T* p, int col, int n, uint32_t voff, T v)
+{
+    const __amdgpu_buffer_rsrc_t r = col_rsrc(p, col, n);
+    if constexpr (sizeof(T) == 8) {
+        using u2 = unsigned __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, 0, 0);
+    } else {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, 0, 0);
+    }
+}
+
+// One barrier interval (S time steps t0 .. t0 + S - 1) of wave g.  Every wave runs the
+// same code at every interval: the first wave's Q loads and the last wave's Q stores are
+// range-checked by the hardware, and a middle wave's inputs outside [0, n) need no test
+// (columns past n are zeros that only identity rotations touch; the LDS ring starts
+// zeroed, so an input slot its producer has not written yet also reads 0).
+template <typename T, class Gm, bool FIRST, bool LAST, bool GENERAL, int D>
+__device__ __forceinline__ void interval(T* At, uint32_t voff, int64_t lda, int n,
+                                         T (&win)[Gm::W], T (&pf)[FIRST ? Gm::PFD : 1][Gm::S],
+                                         const T* cbw, T sg,
+                                         const T* lds_in, T* lds_out, int t0)
+{
+    constexpr int S = Gm::S, W = Gm::W, KG = Gm::KG, PFD = Gm::PFD, NR = Gm::NR;
+    constexpr int SPR = Gm::SPR;
+    // Reads of LDS (this wave's input columns and coefficient rows) are issued LA steps
+    // before the step that uses them.  Issuing them all at the start of the interval
+    // makes every wave of the workgroup queue its whole interval's reads at the LDS right
+    // after the barrier, so the last wave in the queue waits for all of them before its
+    // first FMA.
+    constexpr int LA = 2;
     T in[FIRST ? 1 : S];
+    T cf[NR];
+    auto read_in = [&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        if constexpr (!FIRST && s < S) in[s] = lds_in[s * kLanes];
+        if constexpr (s < S && s % SPR == 0) cf[s / SPR] = cbw[(s / SPR) * 16];
+    };
+    Steps<LA, 0>::run(read_in);
     // first wave: prefetch slot D holds this interval's columns (loaded PFD intervals
     // ago); each is refilled, right after its step consumed it, with the column PFD
     // intervals ahead (t0 + PFD S + 1 + s)
     const int c0f = t0 + PFD * S + 1;
-    const T* __restrict__ pfp = Ar + (int64_t)c0f * lda;
-    if constexpr (!FIRST) {
-        const int c0 = t0 - 2 * g * KG + 1;
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            if constexpr (CHECKED)
-                in[s] = (c0 + s >= 0 && c0 + s < n) ? lds_in[s * kLanes] : 0.0;
-            else
-                in[s] = lds_in[s * kLanes];
-        }
-    }
-    const int o0 = t0 - 2 * Gm::Kp + 2;
-    T* __restrict__ po = Ar + (int64_t)o0 * lda;
+    const int o0 = t0 - 2 * Gm::Kp + 2;  // last wave: column leaving at step 0
+    // running column pointers (uniform; one scalar add per step)
+    T* pq = At + (int64_t)(FIRST ? c0f : o0) * lda;
     Steps<S, 0>::run([&](auto sc) {
         constexpr int s = decltype(sc)::value;
-        constexpr int row = s / Gm::SPR, base = (s % Gm::SPR) * Gm::NC;
+        constexpr int row = s / SPR, base = (s % SPR) * Gm::NC;
+        // Progress-based priority: 3 in the first quarter of the interval down to 0 in
+        // the last, so that of two waves sharing a SIMD the one that is behind gets the
+        // issue slots (the arbiter otherwise favours the older wave, which then reaches
+        // the barrier early and leaves the younger one to run alone).
+        if constexpr (s % (S / 4) == 0) __builtin_amdgcn_s_setprio(3 - s / (S / 4));
+        read_in(std::integral_constant<int, s + LA>{});
         if constexpr (FIRST) {
+            // the column enters the pass with its final sign (see k_colsigns)
             win[(s + 1) % W] = pf[D][s];
-            if constexpr (CHECKED)
-                pf[D][s] = (c0f + s < n) ? pfp[(int64_t)s * lda] : 0.0;
-            else
-                pf[D][s] = pfp[(int64_t)s * lda];
+            if constexpr (!GENERAL) fix_sign<T, s>(win[(s + 1) % W], sg);
+            pf[D][s] = col_load<T>(pq, c0f + s, n, voff);
+            pq += lda;
         } else {
             win[(s + 1) % W] = in[s];
         }
@@ -386,15 +450,12 @@ __device__ __forceinline__ void interval(T* __restrict__ Ar, int64_t lda, int n,
             step_shear<T, KG, s, base>(win, cf[row]);
         T out = win[(s + 2) % W];
         if constexpr (LAST) {
-            if constexpr (!GENERAL) fix_sign<T, s>(out, sg);
-            if constexpr (CHECKED) {
-                if (o0 + s >= 0 && o0 + s < n) po[(int64_t)s * lda] = out;
-            } else {
-                po[(int64_t)s * lda] = out;
-            }
+            col_store<T>(pq, o0 + s, n, voff, out);
+            pq += lda;
         } else {
             lds_out[s * kLanes] = out;
         }
+        __builtin_amdgcn_sched_barrier(0);
     });
 }
 
@@ -460,19 +521,20 @@ __device__ __forceinline__ void store_block(T* cb, const T (&v)[CoefBlock<Gm>::P
 
 template <typename T, class Gm, bool FIRST, bool LAST, bool GENERAL>
 __device__ __forceinline__ void wave(T* __restrict__ A, int64_t lda, int m, int n,
-                                     const T* __restrict__ tab, int Up, int PADU,
-                                     int T_out, int NI,
+                                     const T* __restrict__ tab, const T* __restrict__ sigm1,
+                                     int Up, int PADU, int NI,
                                      T (*ring)[2][Gm::S][kLanes],
                                      T (*cbuf)[CoefBlock<Gm>::SIZE], int g, int lane)
 {
     using CB = CoefBlock<Gm>;
-    constexpr int S = Gm::S, W = Gm::W, KG = Gm::KG, PFD = Gm::PFD, NR = Gm::NR;
+    constexpr int S = Gm::S, W = Gm::W, PFD = Gm::PFD, NR = Gm::NR;
     constexpr bool LOADER = !FIRST && !LAST;
     const int loader = g - 1;
     // Lanes past the last row alias row m-1: they load, compute and store exactly the
     // values of that row's own lane, so no per-lane predication is needed.
-    const int64_t row = min((int64_t)blockIdx.x * kLanes + lane, (int64_t)m - 1);
-    T* __restrict__ Ar = A + row;
+    const int64_t row0 = (int64_t)blockIdx.x * kLanes;
+    T* At = A + row0;  // the tile's first row (uniform)
+    const uint32_t voff = (uint32_t)(min(row0 + lane, (int64_t)m - 1) - row0) * sizeof(T);
     const int cl = lane & 15;  // lane within the 16-lane DPP row
 
     T win[W];
@@ -481,14 +543,24 @@ __device__ __forceinline__ void wave(T* __restrict__ A, int64_t lda, int m, int 
     T pf[FIRST ? PFD : 1][S];
     if constexpr (FIRST) {
         // column 0 enters before step 0 (slot 0); interval d takes columns dS+1 .. dS+S
-        win[0] = Ar[0];
+        win[0] = col_load<T>(At, 0, n, voff);
+        if constexpr (!GENERAL) win[0] = fma(sigm1[0], win[0], win[0]);  // its final sign
 #pragma unroll
         for (int d = 0; d < PFD; ++d)
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 const int col = d * S + s + 1;
-                pf[d][s] = (col < n) ? Ar[(int64_t)col * lda] : 0.0;
+                pf[d][s] = col_load<T>(At + (int64_t)col * lda, col, n, voff);
             }
+    }
+    if constexpr (!LAST) {
+        // zero both buffers of this wave's output ring (read before first written, see
+        // interval())
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            ring[g][0][s][lane] = 0.0;
+            ring[g][1][s][lane] = 0.0;
+        }
     }
     T blk[CB::PER];
     int off[CB::PER];
@@ -508,35 +580,18 @@ __device__ __forceinline__ void wave(T* __restrict__ A, int64_t lda, int m, int 
             if constexpr (LOADER) {
                 if (b + 1 < NI) fetch_block<T, Gm>(blk, off, tab, b + 1);
             }
+            // Every wave runs every interval, also before its first column arrives and
+            // after its last one left: those intervals move zeros through identity
+            // rotations (their table rows are in range and hold zeros), store nothing
+            // (range check) and keep the window in the same registers on every path.
             const int t0 = (b - g) * S + g;
-            const bool active = (t0 + S > 0) && (t0 < T_out);
-            if (active) {
-                const T* cb = cbuf[b & 1];
-                T cf[NR];
-#pragma unroll
-                for (int r = 0; r < NR; ++r) cf[r] = cb[(g * NR + r) * 16 + cl];
-                const T sg = LAST ? cb[CB::ROWS + cl] : 0.0;
-                const T* lds_in =
-                    FIRST ? nullptr : &ring[g > 0 ? g - 1 : 0][(b - 1) & 1][0][lane];
-                T* lds_out = LAST ? nullptr : &ring[g][b & 1][0][lane];
-                bool interior;
-                if constexpr (FIRST) {
-                    interior = t0 + (PFD + 1) * S < n;
-                } else {
-                    const int c0 = t0 - 2 * g * KG + 1;
-                    interior = c0 >= 0 && c0 + S <= n;
-                }
-                if constexpr (LAST) {
-                    const int o0 = t0 - 2 * Gm::Kp + 2;
-                    interior = interior && o0 >= 0 && o0 + S <= n;
-                }
-                if (interior)
-                    interval<T, Gm, FIRST, LAST, false, GENERAL, D>(Ar, lda, n, win, pf, cf, sg,
-                                                                 lds_in, lds_out, t0, g);
-                else
-                    interval<T, Gm, FIRST, LAST, true, GENERAL, D>(Ar, lda, n, win, pf, cf, sg,
-                                                                lds_in, lds_out, t0, g);
-            }
+            const T* cb = cbuf[b & 1];
+            const T* cbw = cb + g * NR * 16 + cl;  // this wave's coefficient rows
+            const T sg = FIRST ? cb[CB::ROWS + cl] : 0.0;  // input signs
+            const T* lds_in = FIRST ? nullptr : &ring[g > 0 ? g - 1 : 0][(b - 1) & 1][0][lane];
+            T* lds_out = LAST ? nullptr : &ring[g][b & 1][0][lane];
+            interval<T, Gm, FIRST, LAST, GENERAL, D>(At, voff, lda, n, win, pf, cbw, sg, lds_in,
+                                                     lds_out, t0);
             if constexpr (LOADER) {
                 if (b + 1 < NI) store_block<T, Gm>(cbuf[(b + 1) & 1], blk, loader, lane);
             }
@@ -548,8 +603,8 @@ __device__ __forceinline__ void wave(T* __restrict__ A, int64_t lda, int m, int 
 template <typename T, class Gm>
 __global__ void __launch_bounds__(kLanes * Gm::G)
     k_apply_dpp(T* __restrict__ A, int64_t lda, int m, int n,
-                const T* __restrict__ tab, const int* __restrict__ flag, int Up, int PADU,
-                int T_out, int NI)
+                const T* __restrict__ tab, const T* __restrict__ sigm1,
+                const int* __restrict__ flag, int Up, int PADU, int NI)
 {
     constexpr int G = Gm::G;
     static_assert(G >= 3, "the coefficient loaders are the middle waves");
@@ -560,13 +615,13 @@ __global__ void __launch_bounds__(kLanes * Gm::G)
     const bool general = *flag != 0;
 #define SEP_WAVE(GEN)                                                                         \
     if (g == 0)                                                                               \
-        wave<T, Gm, true, false, GEN>(A, lda, m, n, tab, Up, PADU, T_out, NI, ring, cbuf, g,     \
+        wave<T, Gm, true, false, GEN>(A, lda, m, n, tab, sigm1, Up, PADU, NI, ring, cbuf, g,     \
                                    lane);                                                     \
     else if (g == G - 1)                                                                      \
-        wave<T, Gm, false, true, GEN>(A, lda, m, n, tab, Up, PADU, T_out, NI, ring, cbuf, g,     \
+        wave<T, Gm, false, true, GEN>(A, lda, m, n, tab, sigm1, Up, PADU, NI, ring, cbuf, g,     \
                                    lane);                                                     \
     else                                                                                      \
-        wave<T, Gm, false, false, GEN>(A, lda, m, n, tab, Up, PADU, T_out, NI, ring, cbuf, g,    \
+        wave<T, Gm, false, false, GEN>(A, lda, m, n, tab, sigm1, Up, PADU, NI, ring, cbuf, g,    \
                                     lane);
     if (general) {
         SEP_WAVE(true)
@@ -604,7 +659,7 @@ int run_dpp(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T
     int tp_prev = tp0;
     for (int64_t L0 = 0; L0 < k; L0 += Kp) {
         hipLaunchKernelGGL(k_colsigns<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cs,
-                           ld_cs, (int)n, (int)k, (int)L0, Kp, flag, mask, sigm1);
+                           ld_cs, (int)n, (int)k, (int)L0, Kp, 1, flag, mask, sigm1);
         SEP_CHECK_LAUNCH();
         const int64_t total = (int64_t)G * Up * KG + (int64_t)NI * 16;
         const int tblocks = (int)std::min<int64_t>((total + 255) / 256, 8192);
@@ -615,7 +670,7 @@ int run_dpp(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T
         const int ta = timing_mark(h, st);
         timing_pair(h, false, tp_prev, ta);
         hipLaunchKernelGGL((k_apply_dpp<T, Gm>), dim3((unsigned)rb), dim3(kLanes * G), 0, st, A,
-                           lda, (int)m, (int)n, w.table.as<T>(), flag, Up, PADU, T_out, NI);
+                           lda, (int)m, (int)n, w.table.as<T>(), sigm1, flag, Up, PADU, NI);
         SEP_CHECK_LAUNCH();
         tp_prev = timing_mark(h, st);
         timing_pair(h, true, ta, tp_prev);
@@ -916,7 +971,7 @@ int run_r16(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T
     int tp_prev = tp0;
     for (int64_t L0 = 0; L0 < k; L0 += Kp) {
         hipLaunchKernelGGL(k_colsigns<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cs,
-                           ld_cs, (int)n, (int)k, (int)L0, Kp, flag, mask, sigm1);
+                           ld_cs, (int)n, (int)k, (int)L0, Kp, 0, flag, mask, sigm1);
         SEP_CHECK_LAUNCH();
         const int tblocks = (int)std::min<int64_t>(((int64_t)tab_elems + 255) / 256, 8192);
         hipLaunchKernelGGL((k_coef_r16<T, Gm>), dim3(tblocks), dim3(256), 0, st,
