@@ -240,6 +240,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--k", type=int, default=64)
+    ap.add_argument("--dtype", choices=("f64", "f32"), default="f64",
+                    help="element type of the apply (the headline is f64; f32 is for "
+                         "profiling the Float32 kernel)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-large", action="store_true",
@@ -270,10 +273,12 @@ def main() -> int:
 
     n = m = args.n
     k = args.k
-    log(f"[bench] workload: bulk Givens apply m=n={n} k={k} f64")
+    f64 = args.dtype == "f64"
+    tdt = torch.float64 if f64 else torch.float32
+    log(f"[bench] workload: bulk Givens apply m=n={n} k={k} {args.dtype}")
     gen = torch.Generator(device="cuda").manual_seed(1)
-    A = torch.randn((n, m), generator=gen, dtype=torch.float64, device="cuda").t()  # column-major
-    grid = torch.from_numpy(sep.inputs.givens_grid(n, k, seed=2)).cuda()
+    A = torch.randn((n, m), generator=gen, dtype=tdt, device="cuda").t()  # column-major
+    grid = torch.from_numpy(sep.inputs.givens_grid(n, k, seed=2)).to(tdt).cuda()
     for _ in range(args.warmup):
         sep.bulk_wave_order_2x2_rmul_(A, grid, handle=h)
     torch.cuda.synchronize()
@@ -295,8 +300,10 @@ def main() -> int:
     # finite check on the result (rotations preserve row norms)
     assert torch.isfinite(A).all().item()
 
+    peak = FP64_PEAK_TFLOPS if f64 else FP32_PEAK_TFLOPS
+    esz = 8 if f64 else 4
     out = {
-        "metric": "Givens-apply GFLOP/s n=16384 FP64",
+        "metric": f"Givens-apply GFLOP/s n={n} " + ("FP64" if f64 else "FP32"),
         "value": round(value, 2),
         "unit": "GFLOP/s",
         "n_gpus": 1,
@@ -306,28 +313,29 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": args.dtype,
         "data": "synthetic (Q ~ N(0,1) seeded on device; theta ~ U[0,2pi) seeded, c=cos, s=sin)",
         "config": {"workload": f"bulk Givens apply (bench(Float64, {m}, {n}, {k}) analogue)",
                    "m": m, "n": n, "k": k, "flops_per_step": flops},
         "roofline": {
             "bound": "mfma",
-            "pipe": "FP64 VALU (v_fmac_f64 with DPP row_newbcast coefficients); MI355X FP64 "
-                    "vector and matrix peaks are both 78.6 TFLOP/s",
+            "pipe": ("FP64 VALU (v_fmac_f64 with DPP row_newbcast coefficients); MI355X FP64 "
+                     "vector and matrix peaks are both 78.6 TFLOP/s") if f64 else
+                    "FP32 VALU (v_fmac_f32 with DPP row_newbcast coefficients)",
             "achieved": round(achieved, 3),
-            "peak": FP64_PEAK_TFLOPS,
+            "peak": peak,
             "unit": "TFLOP/s",
-            "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-            "traffic": pmc_traffic_bytes(),
+            "frac": round(achieved / peak, 4),
+            "traffic": pmc_traffic_bytes() if (f64 and n == 16384 and k == 64) else None,
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE x 1)",
-            "algorithmic_bytes_per_launch": 2.0 * 8 * m * n + 16.0 * (n - 1) * k,
+            "algorithmic_bytes_per_launch": 2.0 * esz * m * n + 2.0 * esz * (n - 1) * k,
             "kernel": "k_apply_dpp<8,16>",
             "kernel_ms": round(kern_ms, 4),
             "algorithmic_flops_per_launch": flops * args.steps / max(launches, 1),
         },
     }
 
-    if not args.no_cpu:
+    if not args.no_cpu and f64:
         threads = min(16, os.cpu_count() or 1)
         log(f"[bench] cpu baseline: oracle wave-order apply, {args.cpu_rows} rows x n={n}, "
             f"k={k}, {threads} threads, best of {args.cpu_reps}")
