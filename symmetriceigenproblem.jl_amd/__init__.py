@@ -362,7 +362,9 @@ def _host_i64(a) -> np.ndarray:
 
 
 def qr_batched_(ns, off_d, off_q, dv, ev, Q, max_layers: int = 0, handle: Optional[Handle] = None):
-    """Solve B problems (Float64, device tensors) with the round-based batched solver.
+    """Solve B problems (Float64, device tensors) with the round-based batched solver
+    (rounds replayed from a captured hipGraph; SEP_BATCHED_GRAPH=0 at handle creation
+    launches them one by one).
 
     Problem p: order ns[p] (even), dv/ev at off_d[p], Q (ns[p]^2, column-major) at off_q[p].
     Returns (converged[B] bool numpy, iters[B] int64 numpy)."""

@@ -40,6 +40,7 @@ constexpr int kLanes = 64;
 constexpr int kStageSteps = 16;  // wave-order time steps staged per LDS fill
 constexpr int kQAhead = 8;       // Q columns prefetched ahead
 constexpr int kMaxBatchedN = 8192;
+constexpr int kGraphRounds = 4;  // rounds per captured graph (even: buffer parity repeats)
 
 // state word (beyond the chase's own) marking a finished problem
 constexpr int ST_DONE = 10;
@@ -338,7 +339,9 @@ int ensure_batch_sync(sep_handle* h)
 {
     if (!h->batch_pinned)
         SEP_HIP(hipHostMalloc((void**)&h->batch_pinned, sizeof(int) * 4, hipHostMallocDefault));
-    for (int i = 0; i < 4; ++i)
+    for (auto& bs : h->batch_stream)
+        if (!bs) SEP_HIP(hipStreamCreateWithFlags(&bs, hipStreamNonBlocking));
+    for (int i = 0; i < 7; ++i)
         if (!h->batch_ev[i]) SEP_HIP(hipEventCreateWithFlags(&h->batch_ev[i], hipEventDisableTiming));
     return SEP_OK;
 }
@@ -385,50 +388,109 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
         s[chase::ST_LBEA] = n[p];
         s[chase::ST_ITER] = 0;
     }
-    hipStream_t cs = h->chase_stream, as = h->stream;
+    hipStream_t us = h->stream;  // the caller's stream: uploads, results
+    hipStream_t as = h->batch_stream[0], cs = h->batch_stream[1];
     int* pin = h->batch_pinned;
     pin[0] = 0;
-    SEP_HIP(hipMemcpyAsync(d_probs, L.probs.data(), probs_b, hipMemcpyHostToDevice, as));
-    SEP_HIP(hipMemcpyAsync(d_work, L.work.data(), work_b, hipMemcpyHostToDevice, as));
-    SEP_HIP(hipMemcpyAsync(d_st, h->batch_host_st.data(), st_b, hipMemcpyHostToDevice, as));
-    SEP_HIP(hipMemcpyAsync(d_count, pin, sizeof(int), hipMemcpyHostToDevice, as));
-    SEP_HIP(hipStreamSynchronize(as));  // also orders the caller's dv / ev / Q before the chase
+    SEP_HIP(hipMemcpyAsync(d_probs, L.probs.data(), probs_b, hipMemcpyHostToDevice, us));
+    SEP_HIP(hipMemcpyAsync(d_work, L.work.data(), work_b, hipMemcpyHostToDevice, us));
+    SEP_HIP(hipMemcpyAsync(d_st, h->batch_host_st.data(), st_b, hipMemcpyHostToDevice, us));
+    SEP_HIP(hipMemcpyAsync(d_count, pin, sizeof(int), hipMemcpyHostToDevice, us));
+    SEP_HIP(hipStreamSynchronize(us));  // also orders the caller's dv / ev / Q before the chase
 
     hipEvent_t* chased = h->batch_ev;     // [2] by round parity
     hipEvent_t* applied = h->batch_ev + 2;
     const int nwork = (int)L.work.size();
     const int64_t max_rounds = 100 * (int64_t)L.nmax + 3;
-    int64_t round = 0;
-    int done = 0;
-    int rc = SEP_OK;
-    for (; round < max_rounds && done < B; ++round) {
-        const int b = (int)(round & 1);
-        if (round >= 2 && hipStreamWaitEvent(cs, applied[b], 0) != hipSuccess) {
-            rc = SEP_ERR_HIP;
-            break;
-        }
-        const unsigned reserve = cfg.CW >= 4 ? 0u : (unsigned)(kLdsPerCU / (4 * cfg.CW));
-        hipLaunchKernelGGL(k_chase_all, dim3((unsigned)B), dim3(kLanes), reserve, cs, d_probs, dv, ev,
-                           d_lt[b], d_grid[b], d_depth[b], d_st, cfg.ML, L.nmax, d_count);
-        if (hipGetLastError() != hipSuccess ||
-            hipMemcpyAsync(pin, d_count, sizeof(int), hipMemcpyDeviceToHost, cs) != hipSuccess ||
-            hipEventRecord(chased[b], cs) != hipSuccess ||
-            hipStreamWaitEvent(as, chased[b], 0) != hipSuccess) {
-            rc = SEP_ERR_HIP;
-            break;
-        }
+    const unsigned reserve = cfg.CW >= 4 ? 0u : (unsigned)(kLdsPerCU / (4 * cfg.CW));
+    // Round r: the chase (chase stream) waits for the apply of round r - 2, which used the
+    // same buffers; the apply (apply stream) waits for the chase; the finished count is
+    // copied to pinned host memory after the chase.  `first` marks the rounds whose
+    // buffers no earlier round of the same sequence used.
+    auto enqueue_round = [&](int64_t r, bool first) -> hipError_t {
+        const int b = (int)(r & 1);
+        hipError_t e = hipSuccess;
+        if (!first && (e = hipStreamWaitEvent(cs, applied[b], 0)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_chase_all, dim3((unsigned)B), dim3(kLanes), reserve, cs, d_probs,
+                           dv, ev, d_lt[b], d_grid[b], d_depth[b], d_st, cfg.ML, L.nmax, d_count);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(pin, d_count, sizeof(int), hipMemcpyDeviceToHost, cs)) !=
+                hipSuccess ||
+            (e = hipEventRecord(chased[b], cs)) != hipSuccess ||
+            (e = hipStreamWaitEvent(as, chased[b], 0)) != hipSuccess)
+            return e;
         if (cfg.KF == 8 && cfg.R == 2)
             launch_apply<8, 2>(nwork, as, d_work, d_probs, d_lt[b], d_grid[b], d_depth[b], Q);
         else if (cfg.KF == 8)
             launch_apply<8, 1>(nwork, as, d_work, d_probs, d_lt[b], d_grid[b], d_depth[b], Q);
         else
             launch_apply<4, 1>(nwork, as, d_work, d_probs, d_lt[b], d_grid[b], d_depth[b], Q);
-        if (hipGetLastError() != hipSuccess || hipEventRecord(applied[b], as) != hipSuccess ||
-            hipEventSynchronize(chased[b]) != hipSuccess) {
-            rc = SEP_ERR_HIP;
-            break;
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        return hipEventRecord(applied[b], as);
+    };
+    int64_t round = 0;
+    int done = 0;
+    int rc = SEP_OK;
+    if (h->batch_eager) {
+        // one round at a time; the host reads the finished count after each chase
+        for (; round < max_rounds && done < B; ++round) {
+            if (enqueue_round(round, round < 2) != hipSuccess ||
+                hipEventSynchronize(chased[round & 1]) != hipSuccess) {
+                rc = SEP_ERR_HIP;
+                break;
+            }
+            done = pin[0];
         }
-        done = pin[0];
+    } else {
+        // kGraphRounds rounds captured once into a graph (both streams: the chase stream
+        // forks from and joins the apply stream inside the capture) and replayed until the
+        // finished count reaches B.  Finished problems return at once from the chase and
+        // leave nothing to apply, so rounds past the end cost two near-empty launches.
+        // Successive replays on the apply stream are ordered, which supplies the
+        // cross-replay dependency of the first two rounds of a replay.
+        const std::vector<int64_t> key = {
+            (int64_t)dv, (int64_t)ev, (int64_t)Q, (int64_t)d_probs, (int64_t)d_work,
+            (int64_t)d_st, (int64_t)d_depth[0], (int64_t)d_lt[0], (int64_t)d_grid[0],
+            (int64_t)d_lt[1], (int64_t)d_grid[1], (int64_t)d_count, (int64_t)pin, B, nwork,
+            L.nmax, cfg.ML, cfg.KF, cfg.R, cfg.CW, (int64_t)cs, (int64_t)as};
+        if (!h->batch_graph || h->batch_graph_key != key) {
+            if (h->batch_graph) {
+                (void)hipGraphExecDestroy(h->batch_graph);
+                h->batch_graph = nullptr;
+            }
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamBeginCapture(as, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) e = hipEventRecord(h->batch_ev[4], as);
+            if (e == hipSuccess) e = hipStreamWaitEvent(cs, h->batch_ev[4], 0);
+            for (int r = 0; r < kGraphRounds && e == hipSuccess; ++r) e = enqueue_round(r, r < 2);
+            if (e == hipSuccess) e = hipEventRecord(h->batch_ev[5], cs);
+            if (e == hipSuccess) e = hipStreamWaitEvent(as, h->batch_ev[5], 0);
+            const hipError_t ee = hipStreamEndCapture(as, &graph);
+            if (e == hipSuccess) e = ee;
+            if (e == hipSuccess) e = hipGraphInstantiate(&h->batch_graph, graph, nullptr, nullptr, 0);
+            if (graph) (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                h->batch_graph = nullptr;
+                set_error("batched: graph capture failed: %s", hipGetErrorString(e));
+                return SEP_ERR_HIP;
+            }
+            h->batch_graph_key = key;
+        }
+        // one replay in flight ahead of the one whose count the host reads
+        hipEvent_t ran = h->batch_ev[6];  // end-of-replay marker (never captured)
+        if (hipGraphLaunch(h->batch_graph, as) != hipSuccess) rc = SEP_ERR_HIP;
+        round = kGraphRounds;
+        while (rc == SEP_OK && round < max_rounds) {
+            if (hipEventRecord(ran, as) != hipSuccess ||
+                hipGraphLaunch(h->batch_graph, as) != hipSuccess ||
+                hipEventSynchronize(ran) != hipSuccess) {
+                rc = SEP_ERR_HIP;
+                break;
+            }
+            round += kGraphRounds;
+            done = pin[0];  // at least the count after the replay that just ended
+            if (done >= B) break;
+        }
     }
     if (hipStreamSynchronize(as) != hipSuccess || hipStreamSynchronize(cs) != hipSuccess)
         rc = SEP_ERR_HIP;
@@ -436,13 +498,14 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
         set_error("batched: a HIP call failed in round %lld", (long long)round);
         return rc;
     }
+    done = pin[0];
     if (done < B) {
         set_error("batched: %d of %lld problems unfinished after %lld rounds", (int)(B - done),
                   (long long)B, (long long)round);
         return SEP_ERR_HIP;
     }
-    SEP_HIP(hipMemcpyAsync(h->batch_host_st.data(), d_st, st_b, hipMemcpyDeviceToHost, as));
-    SEP_HIP(hipStreamSynchronize(as));
+    SEP_HIP(hipMemcpyAsync(h->batch_host_st.data(), d_st, st_b, hipMemcpyDeviceToHost, us));
+    SEP_HIP(hipStreamSynchronize(us));
     std::vector<int32_t> conv((size_t)B);
     std::vector<int64_t> its((size_t)B);
     int all = 1;
@@ -454,10 +517,10 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
     }
     if (converged)
         SEP_HIP(hipMemcpyAsync(converged, conv.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice,
-                               as));
+                               us));
     if (iters)
-        SEP_HIP(hipMemcpyAsync(iters, its.data(), sizeof(int64_t) * B, hipMemcpyHostToDevice, as));
-    SEP_HIP(hipStreamSynchronize(as));
+        SEP_HIP(hipMemcpyAsync(iters, its.data(), sizeof(int64_t) * B, hipMemcpyHostToDevice, us));
+    SEP_HIP(hipStreamSynchronize(us));
     return all ? SEP_OK : SEP_NOT_CONVERGED;
 }
 

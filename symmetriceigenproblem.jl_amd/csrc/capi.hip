@@ -247,6 +247,7 @@ int sep_create(sep_handle** out, int device)
         for (const auto& g : kGeoms)
             if (std::strcmp(ap, g.name) == 0) h->apply_geom = g.geom;
     }
+    if (const char* bg = std::getenv("SEP_BATCHED_GRAPH")) h->batch_eager = std::strcmp(bg, "0") == 0;
     // Blocking streams: work on them is ordered after earlier work on the legacy default
     // stream and before later work there, so device arrays a caller produced or consumes
     // on the default stream need no extra synchronisation.
@@ -287,6 +288,12 @@ int sep_destroy(sep_handle* h)
     h->bwd_M.release();
     h->bwd_part.release();
     for (auto e : h->tev) (void)hipEventDestroy(e);
+    if (h->batch_graph) (void)hipGraphExecDestroy(h->batch_graph);
+    for (auto bs : h->batch_stream)
+        if (bs) {
+            (void)hipStreamSynchronize(bs);
+            (void)hipStreamDestroy(bs);
+        }
     for (auto e : h->batch_ev)
         if (e) (void)hipEventDestroy(e);
     if (h->batch_pinned) (void)hipHostFree(h->batch_pinned);

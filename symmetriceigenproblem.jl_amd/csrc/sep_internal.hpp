@@ -92,7 +92,17 @@ struct sep_handle {
     // batched solver: host copy of the per-problem state words, autotuner choice
     std::vector<int64_t> batch_host_st;
     int* batch_pinned = nullptr;             // pinned finished-problem count
-    hipEvent_t batch_ev[4] = {nullptr, nullptr, nullptr, nullptr};  // chased[2], applied[2]
+    // chased[2], applied[2], fork and join (used while a graph is captured), end of replay
+    hipEvent_t batch_ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // captured rounds of the batched solver, reused while the key (buffers, layout and
+    // configuration it was captured with) matches
+    hipGraphExec_t batch_graph = nullptr;
+    std::vector<int64_t> batch_graph_key;
+    bool batch_eager = false;  // SEP_BATCHED_GRAPH=0: launch the rounds without a graph
+    // the batched solver's own streams (apply, chase): non-blocking, so that the rounds can
+    // be captured whatever stream the caller set; the solver synchronises the caller's
+    // stream before the rounds and its own streams after them
+    hipStream_t batch_stream[2] = {nullptr, nullptr};
     std::vector<int64_t> tune_key;
     int tune_choice = -1;
     int force_kf = 0, force_r = 0;  // sep_set_batched_config (0: autotuned / default)

@@ -121,3 +121,42 @@ def test_rounds_errors(sep, handle):
     conv, iters = sep.qr_batched_(np.array([], dtype=np.int64), np.array([], dtype=np.int64),
                                   np.array([], dtype=np.int64), z, z, z, handle=handle)
     assert conv.shape == (0,) and iters.shape == (0,)
+
+
+def test_graph_replays_match_eager_rounds(sep, monkeypatch):
+    """The default path replays a captured graph of four rounds until every problem has
+    finished; SEP_BATCHED_GRAPH=0 launches the same rounds one at a time.  Three solves on
+    the graph handle reuse the same device buffers (so the second and third replay the
+    graph captured by the first), with the inputs rewritten by ordinary torch work between
+    them; every solve must equal the eager one bit for bit, and the oracle for a sample."""
+    import torch
+    r = np.random.default_rng(8)
+    ns = (2 * r.integers(1, 100, size=64)).astype(np.int64)
+    ns[:3] = [2, 6, 256]
+    ns, d, e, off_d, off_q = _batch(sep, ns, seed=17)
+    print(f"[batched] graph vs eager, B={len(ns)} nmax={ns.max()}", flush=True)
+    monkeypatch.setenv("SEP_BATCHED_GRAPH", "0")
+    he = sep.Handle(0)
+    monkeypatch.delenv("SEP_BATCHED_GRAPH")
+    hg = sep.Handle(0)
+    try:
+        ref = _solve(sep, he, ns, d, e, off_d, off_q, 16)
+        tot_q = int(off_q[-1] + ns[-1] ** 2)
+        dvd = torch.empty(len(d), dtype=torch.float64, device="cuda")
+        evd = torch.empty(len(e), dtype=torch.float64, device="cuda")
+        Q = torch.empty(tot_q, dtype=torch.float64, device="cuda")
+        for rep in range(3):
+            dvd.copy_(torch.from_numpy(d))
+            evd.copy_(torch.from_numpy(e))
+            Q.zero_()
+            for p, n in enumerate(ns):
+                Q[off_q[p]:off_q[p] + n * n].view(n, n).fill_diagonal_(1.0)
+            torch.cuda.synchronize()
+            conv, iters = sep.qr_batched_(ns, off_d, off_q, dvd, evd, Q, 16, handle=hg)
+            out = (conv, iters, dvd.cpu().numpy(), evd.cpu().numpy(), Q.cpu().numpy())
+            for x, y in zip(out, ref):
+                assert np.array_equal(x, y), rep
+        _check_bitwise(ns, d, e, off_d, off_q, 16, *ref, which=range(0, len(ns), 9))
+    finally:
+        he.close()
+        hg.close()
