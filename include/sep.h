@@ -112,6 +112,9 @@ int sep_qr_algorithm_f32_dev(sep_handle *h, int64_t n, float *dv, float *ev, flo
  * n/off_d/off_q are HOST arrays.  Every problem runs qr_algorithm! semantics (tol = eps,
  * maxiter = 100 n[p]): rounds of one block of sweeps per unfinished problem (one
  * wavefront each) followed by the block's application to Q, until every problem is done.
+ * The rounds are captured into a hipGraph once per buffer set, layout and configuration
+ * and replayed (the environment variable SEP_BATCHED_GRAPH=0 at sep_create launches them
+ * one by one instead).  Q = NULL computes eigenvalues only (no application).
  * The results equal per-problem sep_qr_algorithm_f64 up to the apply's rounding form
  * (here Rotation2 rmul! arithmetic, bit for bit the layer-major product).
  * max_layers = 0 takes the autotuned (or default, 32) block length.  converged[p]
@@ -130,6 +133,13 @@ int sep_tune_batched_f64(sep_handle *h, int64_t B, const int64_t *n, const int64
 /* Force the batched apply's fused depth and rows per lane (8/1, 8/2 or 4/1) for later
  * solves on this handle, overriding the autotuner; 0/0 restores it. */
 int sep_set_batched_config(sep_handle *h, int32_t fused_depth, int32_t rows_per_lane);
+
+/* Accept odd orders in the solves of this handle (allow = 1; 0 restores the default).
+ * No counterpart in the reference, which asserts iseven(n) as a "limitation for now"
+ * (src/bulge_chasing.jl:155): the assertion protects only its wave kernels'
+ * preconditions (src/wave.jl:65-66), which this library's apply does not have.  The
+ * default keeps the reference's behaviour (SEP_ERR_ODD_N). */
+int sep_set_odd_order(sep_handle *h, int32_t allow);
 
 /* ---- Backward of the eigendecomposition (no counterpart in the reference) -----
  * Given eigenvalues lam (n), eigenvectors Q (n x n, column-major, ldq) of the

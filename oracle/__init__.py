@@ -70,7 +70,7 @@ def _declare(lib: C.CDLL, pfx: str) -> None:
         f.argtypes = [_I64, _I64, _P, _I64, _P, _I64, C.c_int]
         f.restype = C.c_int
         f = getattr(lib, pfx + "orc_qr_algorithm" + suf)
-        f.argtypes = [_I64, _P, _P, _P, _I64, _I64, ct, _I64, _P, _P, _P, C.c_int, _P]
+        f.argtypes = [_I64, _P, _P, _P, _I64, _I64, ct, _I64, _P, _P, _P, C.c_int, _P, C.c_int]
         f.restype = C.c_int
         f = getattr(lib, pfx + "orc_chase_block" + suf)
         f.argtypes = [_I64, _P, _P, _I64, ct, _I64, _P, _P, _P]
@@ -160,13 +160,14 @@ def bulk_wave(A: np.ndarray, grid: np.ndarray, threads: int = 1, fast: bool = Fa
 
 
 def qr_algorithm(dv, ev, Q=None, max_layers: int = 128, tol=None, maxiter=None, apply_mode: int = 0,
-                 fast: bool = False):
+                 fast: bool = False, allow_odd: bool = False):
     """qr_algorithm!(H, Q, max_layers, tol, maxiter); mutates dv, ev, Q.
 
     Returns (converged, iter, n_rotations, n_blocks).  apply_mode 0 = layer-major grid,
     1 = wave order, 2 = chronological rotation log, 3 = layer-major with the rotation as
     two fused multiply-adds.  fast=True runs the FMA-contracted build
-    (the timed CPU baseline; parity checks use the default build)."""
+    (the timed CPU baseline; parity checks use the default build).  allow_odd accepts odd
+    n (the reference asserts iseven(n); the chase itself has no parity condition)."""
     dt = dv.dtype
     n = dv.shape[0]
     if tol is None:
@@ -184,7 +185,7 @@ def qr_algorithm(dv, ev, Q=None, max_layers: int = 128, tol=None, maxiter=None, 
         qp = _ptr(Q)
     rc = _fn("orc_qr_algorithm", dt, fast)(n, _ptr(dv), _ptr(ev), qp, ldq, max_layers, tol, maxiter,
                                      C.byref(conv), C.byref(it), C.byref(nrot), apply_mode,
-                                     C.byref(nblk))
+                                     C.byref(nblk), int(bool(allow_odd)))
     if rc == -1:
         raise AssertionError("qr_algorithm!: n must be even")
     if rc != 0:

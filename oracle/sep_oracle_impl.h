@@ -429,16 +429,18 @@ int FN(orc_bulk_wave_parallel)(int64_t m, int64_t N, T *A, int64_t lda, const T 
 /* qr_algorithm!, src/bulge_chasing.jl:146-288.  dv (n), ev (n-1), Q column-major
  * (m rows = n, leading dimension ldq; Q may be NULL for eigenvalues only, which the
  * reference does not offer but which leaves dv/ev identical).  Return value 0 on
- * success, -1 for odd n (the @assert at :155); *converged / *iter as the
+ * success, -1 for odd n (the @assert at :155) unless allow_odd (the assert is the
+ * reference's "limitation for now": nothing but its wave kernels' preconditions
+ * depends on the parity, and apply modes 0, 2 and 3 have none); *converged / *iter as the
  * reference's return tuple; *nrot counts the rotations the chase created.
  * apply_mode: 0 = layer-major sequential grid apply, 1 = wave order (requires the
  * wave preconditions, k even and k <= n-2), 2 = chronological log (no grid),
  * 3 = layer-major with the fused-multiply-add rotation. */
 int FN(orc_qr_algorithm)(int64_t n, T *dv, T *ev, T *Q, int64_t ldq, int64_t max_layers, T tol,
                          int64_t maxiter, int32_t *converged, int64_t *iter_out, int64_t *nrot,
-                         int apply_mode, int64_t *nblocks)
+                         int apply_mode, int64_t *nblocks, int allow_odd)
 {
-    if (n % 2 != 0) return -1; /* :155 */
+    if (n % 2 != 0 && !allow_odd) return -1; /* :155 */
     if (max_layers < 1) return -2;
     int64_t start = 1, to = n, iter = 0;
     T *givens = (T *)malloc(sizeof(T) * 2 * (size_t)(n - 1) * (size_t)max_layers);

@@ -53,6 +53,7 @@ EXPORTS = (
     "sep_qr_algorithm_f64", "sep_qr_algorithm_f32", "sep_qr_algorithm_f64_dev",
     "sep_qr_algorithm_f32_dev",
     "sep_qr_batched_f64_dev", "sep_tune_batched_f64", "sep_set_batched_config",
+    "sep_set_odd_order",
     "sep_eig_backward_f64_dev", "sep_eig_backward_batched_f64_dev",
 )
 
@@ -105,6 +106,7 @@ def _declare(L: C.CDLL) -> None:
     L.sep_qr_batched_f64_dev.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _I64, _P, _P]
     L.sep_tune_batched_f64.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P]
     L.sep_set_batched_config.argtypes = [_P, C.c_int32, C.c_int32]
+    L.sep_set_odd_order.argtypes = [_P, C.c_int32]
     L.sep_eig_backward_f64_dev.argtypes = [_P, _I64, _P, _P, _I64, _P, _P, _P, _P]
     L.sep_eig_backward_batched_f64_dev.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P]
     for name in EXPORTS:
@@ -161,6 +163,11 @@ class Handle:
     def set_batched_config(self, fused_depth: int = 0, rows_per_lane: int = 0) -> None:
         """Force the batched apply's fused depth / rows per lane (8/1, 8/2, 4/1); 0/0 = tuned."""
         _check(lib().sep_set_batched_config(self._h, int(fused_depth), int(rows_per_lane)))
+
+    def set_odd_order(self, allow: bool = True) -> None:
+        """Accept odd orders in this handle's solves (an extension: the reference asserts
+        iseven(n) as a limitation of its wave kernels, src/bulge_chasing.jl:155)."""
+        _check(lib().sep_set_odd_order(self._h, int(bool(allow))))
 
     def get_timing(self) -> Tuple[float, float, int]:
         """(apply-kernel ms, preparation-kernels ms, apply launches) since the last reset."""
@@ -366,7 +373,8 @@ def qr_batched_(ns, off_d, off_q, dv, ev, Q, max_layers: int = 0, handle: Option
     (rounds replayed from a captured hipGraph; SEP_BATCHED_GRAPH=0 at handle creation
     launches them one by one).
 
-    Problem p: order ns[p] (even), dv/ev at off_d[p], Q (ns[p]^2, column-major) at off_q[p].
+    Problem p: order ns[p] (even), dv/ev at off_d[p], Q (ns[p]^2, column-major) at off_q[p];
+    Q may be None (eigenvalues only).
     Returns (converged[B] bool numpy, iters[B] int64 numpy)."""
     import torch
     h = handle or default_handle()
@@ -376,7 +384,8 @@ def qr_batched_(ns, off_d, off_q, dv, ev, Q, max_layers: int = 0, handle: Option
     iters = torch.zeros(B, dtype=torch.int64, device=dv.device)
     rc = lib().sep_qr_batched_f64_dev(h.ptr, B, ns.ctypes.data, off_d.ctypes.data,
                                       off_q.ctypes.data, dv.data_ptr(), ev.data_ptr(),
-                                      Q.data_ptr(), int(max_layers), conv.data_ptr(),
+                                      Q.data_ptr() if Q is not None else None,
+                                      int(max_layers), conv.data_ptr(),
                                       iters.data_ptr())
     _check(rc, allow_not_converged=True)
     torch.cuda.synchronize(dv.device)

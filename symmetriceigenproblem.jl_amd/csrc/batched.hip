@@ -273,7 +273,8 @@ struct Layout {
     int64_t tot_lt = 0, tot_g = 0;
 };
 
-int check_batch(int64_t B, const int64_t* n, const int64_t* off_d, const int64_t* off_q)
+int check_batch(int64_t B, const int64_t* n, const int64_t* off_d, const int64_t* off_q,
+                bool allow_odd)
 {
     if (B < 0 || (B > 0 && (!n || !off_d || !off_q))) {
         set_error("batched: invalid batch description");
@@ -289,7 +290,7 @@ int check_batch(int64_t B, const int64_t* n, const int64_t* off_d, const int64_t
                       (long long)p, (long long)n[p], kMaxBatchedN);
             return SEP_ERR_ARG;
         }
-        if (n[p] % 2) {
+        if (n[p] % 2 && !allow_odd) {
             set_error("batched: problem %lld has odd order %lld (the reference asserts iseven(n))",
                       (long long)p, (long long)n[p]);
             return SEP_ERR_ODD_N;
@@ -419,12 +420,15 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
             (e = hipEventRecord(chased[b], cs)) != hipSuccess ||
             (e = hipStreamWaitEvent(as, chased[b], 0)) != hipSuccess)
             return e;
-        if (cfg.KF == 8 && cfg.R == 2)
+        if (!Q) {
+            // eigenvalues only: nothing to apply (the ordering events stay the same)
+        } else if (cfg.KF == 8 && cfg.R == 2) {
             launch_apply<8, 2>(nwork, as, d_work, d_probs, d_lt[b], d_grid[b], d_depth[b], Q);
-        else if (cfg.KF == 8)
+        } else if (cfg.KF == 8) {
             launch_apply<8, 1>(nwork, as, d_work, d_probs, d_lt[b], d_grid[b], d_depth[b], Q);
-        else
+        } else {
             launch_apply<4, 1>(nwork, as, d_work, d_probs, d_lt[b], d_grid[b], d_depth[b], Q);
+        }
         if ((e = hipGetLastError()) != hipSuccess) return e;
         return hipEventRecord(applied[b], as);
     };
@@ -554,9 +558,9 @@ int qr_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const int64_t* of
                    const int64_t* off_q, double* dv, double* ev, double* Q, int64_t max_layers,
                    int32_t* converged, int64_t* iters)
 {
-    if (int rc = check_batch(B, n, off_d, off_q)) return rc;
+    if (int rc = check_batch(B, n, off_d, off_q, h->allow_odd)) return rc;
     if (B == 0) return SEP_OK;
-    if (!dv || !ev || !Q) {
+    if (!dv || !ev) {  // Q == NULL: eigenvalues only
         set_error("batched: null device pointer");
         return SEP_ERR_ARG;
     }
@@ -598,7 +602,7 @@ int set_batched_config(sep_handle* h, int fused_depth, int rows_per_lane)
 int tune_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d,
                      const int64_t* off_q, const double* dv, const double* ev, int32_t* choice)
 {
-    if (int rc = check_batch(B, n, off_d, off_q)) return rc;
+    if (int rc = check_batch(B, n, off_d, off_q, h->allow_odd)) return rc;
     if (B == 0) {
         if (choice) *choice = 0;
         return SEP_OK;

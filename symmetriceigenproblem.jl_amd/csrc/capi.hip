@@ -131,8 +131,9 @@ int qr_checked(sep_handle* h, int64_t n, T* dv, T* ev, T* Q, int64_t ldq, int64_
         set_error("sep_qr_algorithm: invalid n or null dv/ev");
         return SEP_ERR_ARG;
     }
-    if (n % 2 != 0) {
-        set_error("sep_qr_algorithm: n=%lld must be even (the reference asserts iseven(n))",
+    if (n % 2 != 0 && !h->allow_odd) {
+        set_error("sep_qr_algorithm: n=%lld must be even (the reference asserts iseven(n); "
+                  "sep_set_odd_order lifts this)",
                   (long long)n);
         return SEP_ERR_ODD_N;
     }
@@ -161,8 +162,9 @@ int qr_host(sep_handle* h, int64_t n, T* dv, T* ev, T* Q, int64_t ldq, int64_t m
         set_error("sep_qr_algorithm: invalid n or null dv/ev");
         return SEP_ERR_ARG;
     }
-    if (n % 2 != 0) {
-        set_error("sep_qr_algorithm: n=%lld must be even (the reference asserts iseven(n))",
+    if (n % 2 != 0 && !h->allow_odd) {
+        set_error("sep_qr_algorithm: n=%lld must be even (the reference asserts iseven(n); "
+                  "sep_set_odd_order lifts this)",
                   (long long)n);
         return SEP_ERR_ODD_N;
     }
@@ -457,6 +459,16 @@ int sep_set_batched_config(sep_handle* h, int32_t fused_depth, int32_t rows_per_
         return SEP_ERR_ARG;
     }
     return set_batched_config(h, fused_depth, rows_per_lane);
+}
+
+int sep_set_odd_order(sep_handle* h, int32_t allow)
+{
+    if (!h) {
+        set_error("sep_set_odd_order: null handle");
+        return SEP_ERR_ARG;
+    }
+    h->allow_odd = allow != 0;
+    return SEP_OK;
 }
 
 int sep_eig_backward_f64_dev(sep_handle* h, int64_t n, const double* lam, const double* Q,

@@ -160,3 +160,48 @@ def test_graph_replays_match_eager_rounds(sep, monkeypatch):
     finally:
         he.close()
         hg.close()
+
+
+def test_eigenvalues_only(sep, handle):
+    """Q = None: the same chase without the application.  dv / ev / iteration counts equal
+    the eigenvector solve's bit for bit (the chase never reads Q) and LAPACK's eigenvalues."""
+    import torch
+    r = np.random.default_rng(31)
+    ns = (2 * r.integers(1, 128, size=96)).astype(np.int64)
+    ns, d, e, off_d, off_q = _batch(sep, ns, seed=23)
+    conv, iters, gd, ge, _ = _solve(sep, handle, ns, d, e, off_d, off_q, 32)
+    dvd = torch.from_numpy(d.copy()).cuda()
+    evd = torch.from_numpy(e.copy()).cuda()
+    conv2, iters2 = sep.qr_batched_(ns, off_d, off_q, dvd, evd, None, 32, handle=handle)
+    assert np.array_equal(conv, conv2) and np.array_equal(iters, iters2)
+    assert np.array_equal(dvd.cpu().numpy(), gd) and np.array_equal(evd.cpu().numpy(), ge)
+    for p in range(0, len(ns), 5):
+        n, o = int(ns[p]), off_d[p]
+        ref = sl.eigvalsh_tridiagonal(d[o:o + n], e[o:o + n - 1])
+        assert np.abs(np.sort(gd[o:o + n]) - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_odd_orders_extension(sep):
+    """Odd orders after sep_set_odd_order (rejected by default, like the reference's
+    iseven(n) assertion): bit for bit against the oracle with allow_odd."""
+    h = sep.Handle(0)
+    try:
+        r = np.random.default_rng(41)
+        ns = r.integers(2, 160, size=48).astype(np.int64)
+        ns[:3] = [3, 5, 257]
+        ns, d, e, off_d, off_q = _batch(sep, ns, seed=29)
+        with pytest.raises(AssertionError):
+            _solve(sep, h, ns, d, e, off_d, off_q, 16)
+        h.set_odd_order(True)
+        conv, iters, gd, ge, gQ = _solve(sep, h, ns, d, e, off_d, off_q, 16)
+        for p in range(len(ns)):
+            n, o = int(ns[p]), off_d[p]
+            dv, ev = d[o:o + n].copy(), e[o:o + n - 1].copy()
+            Qo = np.asfortranarray(np.eye(n))
+            oconv, oit, _, _ = oracle.qr_algorithm(dv, ev, Qo, 16, apply_mode=3, allow_odd=True)
+            assert bool(conv[p]) == oconv and iters[p] == oit, (p, n)
+            assert np.array_equal(gd[o:o + n], dv) and np.array_equal(ge[o:o + n - 1], ev)
+            Qg = gQ[off_q[p]:off_q[p] + n * n].reshape(n, n).T
+            assert np.array_equal(Qg, Qo), (p, n)
+    finally:
+        h.close()

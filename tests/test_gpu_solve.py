@@ -187,3 +187,33 @@ def test_single_large_solve_n16384(sep, handle):
     print(f"[solve] n={n} iter={it} residual {res:.3e} orthogonality {orth:.3e}", flush=True)
     assert res <= 1e-13
     assert orth <= 1e-13 * n
+
+
+@pytest.mark.parametrize("n,max_layers", [(3, 128), (9, 4), (201, 16), (513, 64)])
+def test_odd_order_extension(sep, n, max_layers):
+    """sep_set_odd_order lifts the reference's iseven(n) assertion (its wave kernels'
+    limitation; this library's apply has no parity condition).  A fresh handle rejects odd
+    n as the reference does; after set_odd_order the solve equals the oracle's (allow_odd)
+    bit for bit on dv / ev / iteration and rotation counts, Q to the solve tolerance, and
+    LAPACK."""
+    h = sep.Handle(0)
+    try:
+        d, e = sep.inputs.tridiag_linear(n, seed=n)
+        with pytest.raises(AssertionError):
+            sep.qr_algorithm_(sep.SymTridiagonal(d.copy(), e.copy()), None, handle=h)
+        h.set_odd_order(True)
+        od, oe = d.copy(), e.copy()
+        oQ = np.asfortranarray(np.eye(n))
+        oconv, oit, onrot, _ = oracle.qr_algorithm(od, oe, oQ, max_layers, allow_odd=True)
+        H = sep.SymTridiagonal(d.copy(), e.copy())
+        Q = np.asfortranarray(np.eye(n))
+        _, conv, it, nrot = sep.qr_algorithm_(H, Q, max_layers, handle=h, return_rotations=True)
+        assert conv and oconv and it == oit and nrot == onrot
+        assert np.array_equal(H.dv, od) and np.array_equal(H.ev, oe)
+        assert np.linalg.norm(Q - oQ) <= 1e-13 * np.sqrt(n)
+        A = _dense(d, e)
+        assert np.linalg.norm(A @ Q - Q * H.dv) <= 1e-13 * np.linalg.norm(A)
+        ref = sl.eigvalsh_tridiagonal(d, e)
+        assert np.abs(np.sort(H.dv) - ref).max() <= 1e-12 * np.abs(ref).max()
+    finally:
+        h.close()

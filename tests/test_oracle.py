@@ -294,6 +294,38 @@ def test_odd_n_asserts_and_maxiter():
     assert not conv and it == 6
 
 
+@pytest.mark.parametrize("n", [1, 3, 9, 51, 201])
+def test_odd_n_extension_pinned(n):
+    """allow_odd lifts the reference's iseven(n) assertion (an extension; the reference
+    marks it "limitation for now", src/bulge_chasing.jl:155).  Pinned by mathematics:
+    LAPACK eigenvalues and residual / orthogonality on random input, and the Toeplitz
+    closed form lambda_j = a + 2 b cos(j pi / (n + 1)) with its sine eigenvectors."""
+    r = np.random.default_rng(n)
+    d, e = np.arange(1.0, n + 1), r.random(n - 1)
+    dv, ev = d.copy(), e.copy()
+    Q = np.asfortranarray(np.eye(n))
+    conv, it, nrot, _ = oracle.qr_algorithm(dv, ev, Q, 16, allow_odd=True)
+    assert conv
+    if n > 1:
+        ref = sl.eigvalsh_tridiagonal(d, e)
+        assert np.abs(np.sort(dv) - ref).max() <= 1e-12 * np.abs(ref).max()
+        A = dense(d, e)
+        assert np.linalg.norm(A @ Q - Q * dv) <= 1e-13 * np.linalg.norm(A)
+        assert np.linalg.norm(Q.T @ Q - np.eye(n)) <= 1e-13 * n
+    else:
+        assert it == 0 and nrot == 0 and dv[0] == 1.0 and Q[0, 0] == 1.0
+    a, b = 2.0, -1.0
+    dv, ev = np.full(n, a), np.full(n - 1, b)
+    Q = np.asfortranarray(np.eye(n))
+    conv, *_ = oracle.qr_algorithm(dv, ev, Q, allow_odd=True)
+    j = np.arange(1, n + 1)
+    lam = a + 2 * b * np.cos(j * np.pi / (n + 1))
+    assert conv and np.abs(np.sort(dv) - np.sort(lam)).max() <= 1e-13 * 4
+    V = np.sqrt(2 / (n + 1)) * np.sin(np.outer(j, j) * np.pi / (n + 1))
+    Qs, Vs = Q[:, np.argsort(dv)], V[:, np.argsort(lam)]
+    assert np.abs(np.abs(np.sum(Qs * Vs, 0)) - 1).max() <= 1e-10
+
+
 def test_chase_block_resumes_to_full_solve():
     """Running the outer loop block by block through orc_chase_block reproduces the
     whole-solve eigenvalues and iteration count bit for bit."""
