@@ -227,9 +227,11 @@ __device__ __forceinline__ Rot<double> givens<double>(double f, double g)
 {
     const uint32_t fh = __builtin_amdgcn_readfirstlane((int)hi_word(f));
     const uint32_t gh = __builtin_amdgcn_readfirstlane((int)hi_word(g));
-    const uint32_t fa = fh & 0x7fffffffu, ga = gh & 0x7fffffffu;
-    const uint32_t bigh = fa > ga ? fa : ga, smallh = fa < ga ? fa : ga;
-    const bool rare = !(bigh < 0x5E400000u) | !(smallh > 0x28300000u);  // 2^485, 2^-380
+    // hi(|x|) in (hi(2^-380), hi(2^485)) for both, as one unsigned compare of the larger
+    // offset: (h - lo - 1) wraps around for h <= lo
+    constexpr uint32_t kLo = 0x28300000u, kHi = 0x5E400000u;  // 2^-380, 2^485
+    const uint32_t fo = (fh & 0x7fffffffu) - (kLo + 1), go = (gh & 0x7fffffffu) - (kLo + 1);
+    const bool rare = (fo > go ? fo : go) >= kHi - (kLo + 1);
     const bool gt = __ballot(tabs(f) > tabs(g)) != 0;
     const uint32_t sm = (gt && (fh >> 31)) ? 0x80000000u : 0u;
     // square root of t = f^2 + g^2 (as sqrt_common), keeping the half-reciprocal estimate
@@ -456,11 +458,11 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                 oS = mine ? s : oS;
                 p1 = B12pp;
                 d_cur = B22pp;
-                if (i0 + j + 2 <= to) {
-                    const T B23 = rdlane(E, j);
-                    bulge = s * B23;
-                    e_cur = c * B23;
-                }
+                // B23 = ev[i+1]; the last step of the sweep has none: its prefetch slot
+                // holds 0, and the bulge / e_cur it then produces are never read
+                const T B23 = rdlane(E, j);
+                bulge = s * B23;
+                e_cur = c * B23;
             }
             // coalesced stores of the chunk; the next chunk was prefetched from entries
             // these stores do not touch (dv/ev beyond i0 + 62, lt beyond i0 + 62)

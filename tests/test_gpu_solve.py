@@ -217,3 +217,21 @@ def test_odd_order_extension(sep, n, max_layers):
         assert np.abs(np.sort(H.dv) - ref).max() <= 1e-12 * np.abs(ref).max()
     finally:
         h.close()
+
+
+@pytest.mark.parametrize("scale_exp", [600, -600, 470, -370])
+def test_extreme_magnitudes_bitwise(sep, handle, scale_exp):
+    """T scaled by 2^scale_exp: with |f|, |g| beyond (2^-380, 2^485) every rotation takes the
+    complete DLARTG path (power-of-two rescaling); near the bounds (2^470, 2^-370) steps mix
+    both paths.  dv / ev / iteration count bit for bit against the oracle."""
+    n = 120
+    d, e = sep.inputs.tridiag_linear(n, seed=11)
+    d, e = np.ldexp(d, scale_exp), np.ldexp(e, scale_exp)
+    (od, oe, oQ, oconv, oit, onrot), (gd, ge, gQ, conv, it, nrot) = _solve_both(
+        sep, handle, d, e, 16)
+    assert conv and oconv and it == oit and nrot == onrot
+    assert np.array_equal(gd, od) and np.array_equal(ge, oe)
+    assert np.linalg.norm(gQ - oQ) <= 1e-13 * np.sqrt(n)
+    ref = np.ldexp(sl.eigvalsh_tridiagonal(np.ldexp(d, -scale_exp), np.ldexp(e, -scale_exp)),
+                   scale_exp)
+    assert np.abs(np.sort(gd) - ref).max() <= 1e-12 * np.abs(ref).max()
