@@ -232,8 +232,9 @@ __device__ __forceinline__ Rot<double> givens<double>(double f, double g)
     constexpr uint32_t kLo = 0x28300000u, kHi = 0x5E400000u;  // 2^-380, 2^485
     const uint32_t fo = (fh & 0x7fffffffu) - (kLo + 1), go = (gh & 0x7fffffffu) - (kLo + 1);
     const bool rare = (fo > go ? fo : go) >= kHi - (kLo + 1);
+    // sign rule: |f| > |g| (one compare, read as a wave ballot) and f < 0 (fh's sign bit)
     const bool gt = __ballot(tabs(f) > tabs(g)) != 0;
-    const uint32_t sm = (gt && (fh >> 31)) ? 0x80000000u : 0u;
+    const uint32_t sm = (gt ? fh : 0u) & 0x80000000u;
     // square root of t = f^2 + g^2 (as sqrt_common), keeping the half-reciprocal estimate
     const double t = f * f + g * g;
     const double y = __builtin_amdgcn_rsq(t);
@@ -286,6 +287,32 @@ __device__ __forceinline__ T wilkinson(T H11, T H12, T H22)
     T lam = tabs(H22 - l1) < tabs(H22 - l2) ? l1 : l2;
     lam *= scale;
     return lam;
+}
+
+// Lane j of (oR, oD, oC, oS) <- (r, d, c, s) (values the same in every lane): one masked
+// block of 64-bit moves with EXEC = exec & lane mask `m`, restored inside the block.
+__device__ __forceinline__ void capture_lane(double& oR, double& oD, double& oC, double& oS,
+                                             double r, double d, double c, double s,
+                                             uint64_t m)
+{
+    uint64_t save;
+    asm volatile("s_and_saveexec_b64 %4, %5\n"
+                 "v_mov_b64 %0, %6\n"
+                 "v_mov_b64 %1, %7\n"
+                 "v_mov_b64 %2, %8\n"
+                 "v_mov_b64 %3, %9\n"
+                 "s_mov_b64 exec, %4\n"
+                 : "+v"(oR), "+v"(oD), "+v"(oC), "+v"(oS), "=&s"(save)
+                 : "s"(m), "v"(r), "v"(d), "v"(c), "v"(s));
+}
+__device__ __forceinline__ void capture_lane(float& oR, float& oD, float& oC, float& oS, float r,
+                                             float d, float c, float s, uint64_t m)
+{
+    const bool mine = ((m >> (__lane_id() & 63)) & 1) != 0;
+    oR = mine ? r : oR;
+    oD = mine ? d : oD;
+    oC = mine ? c : oC;
+    oS = mine ? s : oS;
 }
 
 // Problem view.  dv (n), ev (n-1), lt (n) and the (c,s) grid.  With tm == 0 the grid is
@@ -451,11 +478,7 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                 const T B11pp = __builtin_fma(B11p, c, B12p * s);
                 const T B12pp = __builtin_fma(B11p, -s, B12p * c);
                 const T B22pp = __builtin_fma(B21p, -s, B22p * c);
-                const bool mine = lane == j;
-                oR = mine ? G.r : oR;
-                oD = mine ? B11pp : oD;
-                oC = mine ? c : oC;
-                oS = mine ? s : oS;
+                capture_lane(oR, oD, oC, oS, G.r, B11pp, c, s, 1ull << j);
                 p1 = B12pp;
                 d_cur = B22pp;
                 // B23 = ev[i+1]; the last step of the sweep has none: its prefetch slot
