@@ -130,7 +130,7 @@ int sep_qr_batched_f64_dev(sep_handle *h, int64_t B, const int64_t *n, const int
 int sep_tune_batched_f64(sep_handle *h, int64_t B, const int64_t *n, const int64_t *off_d,
                          const int64_t *off_q, const double *dv, const double *ev,
                          int32_t *choice);
-/* Force the batched apply's fused depth and rows per lane (8/1, 8/2 or 4/1) for later
+/* Force the batched apply's fused depth and rows per lane (16/1, 8/1, 8/2 or 4/1) for later
  * solves on this handle, overriding the autotuner; 0/0 restores it. */
 int sep_set_batched_config(sep_handle *h, int32_t fused_depth, int32_t rows_per_lane);
 

@@ -161,7 +161,7 @@ class Handle:
         _check(lib().sep_set_timing(self._h, 1 if enable else 0))
 
     def set_batched_config(self, fused_depth: int = 0, rows_per_lane: int = 0) -> None:
-        """Force the batched apply's fused depth / rows per lane (8/1, 8/2, 4/1); 0/0 = tuned."""
+        """Force the batched apply's fused depth / rows per lane (16/1, 8/1, 8/2, 4/1); 0/0 = tuned."""
         _check(lib().sep_set_batched_config(self._h, int(fused_depth), int(rows_per_lane)))
 
     def set_odd_order(self, allow: bool = True) -> None:

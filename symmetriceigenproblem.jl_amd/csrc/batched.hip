@@ -63,10 +63,13 @@ struct BWork {
 // One block of sweeps for every unfinished problem (one wavefront each).  lt / grid /
 // depth are this round's buffers.
 __global__ void __launch_bounds__(kLanes)
-    k_chase_all(const BProb* __restrict__ probs, double* dv, double* ev, int* lt, double* grid,
-                int4* info, int64_t* st_all, int ML, int nmax, int* done_count)
+    k_chase_all(const BProb* __restrict__ probs, const int* __restrict__ order, double* dv,
+                double* ev, int* lt, double* grid, int4* info, int64_t* st_all, int ML, int nmax,
+                int* done_count)
 {
-    const int p = blockIdx.x;
+    // blocks are dispatched in index order as wave slots free up; block b takes the b-th
+    // largest problem, so the longest blocks start first (longest-first list scheduling)
+    const int p = order[blockIdx.x];
     const int lane = threadIdx.x;
     int64_t* st = st_all + (int64_t)p * chase::ST_WORDS;
     if (__builtin_amdgcn_readfirstlane((int)st[ST_DONE])) {
@@ -146,7 +149,8 @@ __global__ void __launch_bounds__(kLanes)
                 const int4* __restrict__ info, double* __restrict__ Q)
 {
     constexpr int W = 2 * KF;
-    constexpr int SB = kStageSteps;
+    // a stage is a whole number of window turns (the window slots are compile-time)
+    constexpr int SB = W > kStageSteps ? W : kStageSteps;
     constexpr int PA = kQAhead;
     constexpr int NE = SB * KF / kLanes;  // staged entries per lane
     static_assert(SB % W == 0 && SB % PA == 0 && NE * kLanes == SB * KF, "stage geometry");
@@ -261,13 +265,16 @@ struct Config {
 };
 constexpr Config kConfigs[] = {
     {32, 8, 2, 4}, {32, 8, 2, 2}, {32, 8, 2, 1}, {64, 8, 2, 2}, {64, 8, 2, 1},
-    {32, 8, 1, 2}, {16, 8, 1, 2}, {64, 8, 1, 1}, {32, 4, 1, 2},
+    {32, 8, 1, 2}, {16, 8, 1, 2}, {64, 8, 1, 1}, {32, 4, 1, 2}, {16, 8, 1, 4},
+    {16, 8, 2, 2}, {16, 8, 1, 1}, {8, 8, 1, 2},  {8, 4, 1, 2},  {24, 8, 1, 2},
+    {16, 16, 1, 4}, {16, 16, 1, 2}, {32, 16, 1, 4}, {32, 16, 1, 2}, {24, 16, 1, 4},
 };
 constexpr int kLdsPerCU = 160 * 1024;
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 
 struct Layout {
     std::vector<BProb> probs;
+    std::vector<int> order;  // problems by decreasing order n (stable)
     std::vector<BWork> work;
     int nmax = 0;
     int64_t tot_lt = 0, tot_g = 0;
@@ -319,7 +326,8 @@ void build_layout(int64_t B, const int64_t* n, const int64_t* off_d, const int64
         L.nmax = std::max(L.nmax, (int)n[p]);
     }
     // largest problems first: their apply is the longest
-    std::vector<int> order(B);
+    std::vector<int>& order = L.order;
+    order.resize(B);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return n[a] > n[b]; });
     const int rows = kLanes * cfg.R;
@@ -361,9 +369,10 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
     const size_t work_b = sizeof(BWork) * L.work.size();
     const size_t st_b = sizeof(int64_t) * chase::ST_WORDS * (size_t)B;
     const size_t depth_b = sizeof(int4) * (size_t)B;
+    const size_t order_b = sizeof(int) * (size_t)B;
     auto al = [](size_t x) { return (x + 255) / 256 * 256; };
     if (int rc = h->batch_meta.reserve(al(probs_b) + al(work_b) + al(st_b) + 2 * al(depth_b) +
-                                       256))
+                                       256 + al(order_b)))
         return rc;
     const size_t lt_set = al(sizeof(int) * (size_t)L.tot_lt);
     const size_t g_set = al(sizeof(double) * 2 * (size_t)L.tot_g);
@@ -376,6 +385,7 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
     d_depth[0] = reinterpret_cast<int4*>(m + al(probs_b) + al(work_b) + al(st_b));
     d_depth[1] = reinterpret_cast<int4*>(reinterpret_cast<char*>(d_depth[0]) + al(depth_b));
     int* d_count = reinterpret_cast<int*>(reinterpret_cast<char*>(d_depth[1]) + al(depth_b));
+    int* d_order = reinterpret_cast<int*>(reinterpret_cast<char*>(d_count) + 256);
     char* sc = static_cast<char*>(h->batch_scratch.p);
     int* d_lt[2] = {reinterpret_cast<int*>(sc), reinterpret_cast<int*>(sc + lt_set)};
     double* d_grid[2] = {reinterpret_cast<double*>(sc + 2 * lt_set),
@@ -395,6 +405,7 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
     pin[0] = 0;
     SEP_HIP(hipMemcpyAsync(d_probs, L.probs.data(), probs_b, hipMemcpyHostToDevice, us));
     SEP_HIP(hipMemcpyAsync(d_work, L.work.data(), work_b, hipMemcpyHostToDevice, us));
+    SEP_HIP(hipMemcpyAsync(d_order, L.order.data(), order_b, hipMemcpyHostToDevice, us));
     SEP_HIP(hipMemcpyAsync(d_st, h->batch_host_st.data(), st_b, hipMemcpyHostToDevice, us));
     SEP_HIP(hipMemcpyAsync(d_count, pin, sizeof(int), hipMemcpyHostToDevice, us));
     SEP_HIP(hipStreamSynchronize(us));  // also orders the caller's dv / ev / Q before the chase
@@ -413,7 +424,8 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
         hipError_t e = hipSuccess;
         if (!first && (e = hipStreamWaitEvent(cs, applied[b], 0)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_chase_all, dim3((unsigned)B), dim3(kLanes), reserve, cs, d_probs,
-                           dv, ev, d_lt[b], d_grid[b], d_depth[b], d_st, cfg.ML, L.nmax, d_count);
+                           d_order, dv, ev, d_lt[b], d_grid[b], d_depth[b], d_st, cfg.ML, L.nmax,
+                           d_count);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = hipMemcpyAsync(pin, d_count, sizeof(int), hipMemcpyDeviceToHost, cs)) !=
                 hipSuccess ||
@@ -422,6 +434,8 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
             return e;
         if (!Q) {
             // eigenvalues only: nothing to apply (the ordering events stay the same)
+        } else if (cfg.KF == 16) {
+            launch_apply<16, 1>(nwork, as, d_work, d_probs, d_lt[b], d_grid[b], d_depth[b], Q);
         } else if (cfg.KF == 8 && cfg.R == 2) {
             launch_apply<8, 2>(nwork, as, d_work, d_probs, d_lt[b], d_grid[b], d_depth[b], Q);
         } else if (cfg.KF == 8) {
@@ -455,6 +469,7 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
         const std::vector<int64_t> key = {
             (int64_t)dv, (int64_t)ev, (int64_t)Q, (int64_t)d_probs, (int64_t)d_work,
             (int64_t)d_st, (int64_t)d_depth[0], (int64_t)d_lt[0], (int64_t)d_grid[0],
+            (int64_t)d_order,
             (int64_t)d_lt[1], (int64_t)d_grid[1], (int64_t)d_count, (int64_t)pin, B, nwork,
             L.nmax, cfg.ML, cfg.KF, cfg.R, cfg.CW, (int64_t)cs, (int64_t)as};
         if (!h->batch_graph || h->batch_graph_key != key) {
@@ -587,10 +602,10 @@ int set_batched_config(sep_handle* h, int fused_depth, int rows_per_lane)
         return SEP_OK;
     }
     const bool ok = (fused_depth == 8 && (rows_per_lane == 1 || rows_per_lane == 2)) ||
-                    (fused_depth == 4 && rows_per_lane == 1);
+                    ((fused_depth == 4 || fused_depth == 16) && rows_per_lane == 1);
     if (!ok) {
         set_error("batched: unsupported configuration fused_depth=%d rows_per_lane=%d "
-                  "(supported: 8/1, 8/2, 4/1; 0/0 restores the autotuned choice)",
+                  "(supported: 16/1, 8/1, 8/2, 4/1; 0/0 restores the autotuned choice)",
                   fused_depth, rows_per_lane);
         return SEP_ERR_ARG;
     }

@@ -52,7 +52,8 @@ def _check_bitwise(ns, d, e, off_d, off_q, ml, conv, iters, gd, ge, gQ, which=No
 
 
 @pytest.mark.parametrize("max_layers,fused_depth,rows_per_lane",
-                         [(0, 0, 0), (16, 8, 1), (20, 8, 2), (64, 8, 2), (16, 4, 1), (7, 4, 1)])
+                         [(0, 0, 0), (16, 8, 1), (20, 8, 2), (64, 8, 2), (16, 4, 1), (7, 4, 1),
+                          (16, 16, 1), (40, 16, 1)])
 def test_rounds_bitwise_equal_to_oracle(sep, handle, max_layers, fused_depth, rows_per_lane):
     r = np.random.default_rng(21)
     ns = (2 * r.integers(1, 70, size=40)).astype(np.int64)
