@@ -117,7 +117,7 @@ int sep_qr_algorithm_f32_dev(sep_handle *h, int64_t n, float *dv, float *ev, flo
  * one by one instead).  Q = NULL computes eigenvalues only (no application).
  * The results equal per-problem sep_qr_algorithm_f64 up to the apply's rounding form
  * (here Rotation2 rmul! arithmetic, bit for bit the layer-major product).
- * max_layers = 0 takes the autotuned (or default, 32) block length.  converged[p]
+ * max_layers = 0 takes the autotuned (or default, 16) block length.  converged[p]
  * (int32) and iters[p] (int64) are written per problem.  Synchronous: returns after the
  * solve.  Returns SEP_NOT_CONVERGED if any problem did not converge. */
 int sep_qr_batched_f64_dev(sep_handle *h, int64_t B, const int64_t *n, const int64_t *off_d,

@@ -143,10 +143,10 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off
 // loads of columns past the range read 0 and the stores of columns outside it (and of
 // lanes past the last row) are dropped without a branch.
 template <int KF, int R>
-__global__ void __launch_bounds__(kLanes)
-    k_apply_all(const BWork* __restrict__ work, const BProb* __restrict__ probs,
-                const int* __restrict__ lt, const double* __restrict__ grid,
-                const int4* __restrict__ info, double* __restrict__ Q)
+__device__ __forceinline__ void apply_item(const BWork wk, const BProb* __restrict__ probs,
+                                           const int* __restrict__ lt,
+                                           const double* __restrict__ grid,
+                                           const int4* __restrict__ info, double* __restrict__ Q)
 {
     constexpr int W = 2 * KF;
     // a stage is a whole number of window turns (the window slots are compile-time)
@@ -156,10 +156,9 @@ __global__ void __launch_bounds__(kLanes)
     static_assert(SB % W == 0 && SB % PA == 0 && NE * kLanes == SB * KF, "stage geometry");
     __shared__ double2 stg[2][SB][KF];
 
-    const BWork wk = work[blockIdx.x];
     const int4 inf = info[wk.prob];
     const int depth = __builtin_amdgcn_readfirstlane(inf.x);
-    if (depth == 0) return;
+    if (depth == 0) return;  // nothing to apply for this item
     const int c0 = __builtin_amdgcn_readfirstlane(inf.y);
     const int nc = __builtin_amdgcn_readfirstlane(inf.z);
     const BProb pr = probs[wk.prob];
@@ -257,17 +256,33 @@ __global__ void __launch_bounds__(kLanes)
     }
 }
 
+// The block's application for every work item, largest problems first.  The grid may be
+// smaller than the work list (each wave then takes items blockIdx.x, + gridDim.x, ...):
+// a capped grid leaves wave slots free for the next round's chase, which runs beside it.
+template <int KF, int R>
+__global__ void __launch_bounds__(kLanes)
+    k_apply_all(const BWork* __restrict__ work, int nwork, const BProb* __restrict__ probs,
+                const int* __restrict__ lt, const double* __restrict__ grid,
+                const int4* __restrict__ info, double* __restrict__ Q)
+{
+    for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x)
+        apply_item<KF, R>(work[wi], probs, lt, grid, info, Q);
+}
+
 // max_layers, fused depth, rows per lane, and chase waves per SIMD (the chase kernel
 // reserves LDS so that at most CW of its waves share a SIMD: a chase wave's serial
 // recurrence slows down in proportion to the chase waves beside it)
 struct Config {
     int ML, KF, R, CW;
+    int AW = 0;  // apply waves per SIMD (0: one wave per work item, uncapped)
 };
 constexpr Config kConfigs[] = {
-    {32, 8, 2, 4}, {32, 8, 2, 2}, {32, 8, 2, 1}, {64, 8, 2, 2}, {64, 8, 2, 1},
-    {32, 8, 1, 2}, {16, 8, 1, 2}, {64, 8, 1, 1}, {32, 4, 1, 2}, {16, 8, 1, 4},
-    {16, 8, 2, 2}, {16, 8, 1, 1}, {8, 8, 1, 2},  {8, 4, 1, 2},  {24, 8, 1, 2},
-    {16, 16, 1, 4}, {16, 16, 1, 2}, {32, 16, 1, 4}, {32, 16, 1, 2}, {24, 16, 1, 4},
+    // measured on the 4096-problem workload (n in [32, 512]): 16 fused layers (one pass
+    // over Q per block of 16) beat 8 (81-95 ms) and 4 (122-127 ms); one chase wave per
+    // SIMD (CW 1) costs 103-108 ms; capping the apply grid (AW) does not help
+    {16, 16, 1, 2}, {16, 16, 1, 4}, {32, 16, 1, 2}, {32, 16, 1, 4}, {24, 16, 1, 4},
+    {16, 16, 1, 4, 1}, {16, 8, 1, 4},  {16, 8, 1, 2},  {8, 8, 1, 2},   {32, 8, 2, 4},
+    {32, 8, 1, 2},  {64, 8, 2, 2},
 };
 constexpr int kLdsPerCU = 160 * 1024;
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
@@ -336,11 +351,12 @@ void build_layout(int64_t B, const int64_t* n, const int64_t* off_d, const int64
 }
 
 template <int KF, int R>
-void launch_apply(int nwork, hipStream_t s, const BWork* w, const BProb* p, const int* lt,
-                  const double* g, const int4* depth, double* Q)
+void launch_apply(int nwork, int grid_cap, hipStream_t s, const BWork* w, const BProb* p,
+                  const int* lt, const double* g, const int4* depth, double* Q)
 {
-    hipLaunchKernelGGL((k_apply_all<KF, R>), dim3(nwork), dim3(kLanes), 0, s, w, p, lt, g, depth,
-                       Q);
+    const int blocks = grid_cap > 0 ? std::min(nwork, grid_cap) : nwork;
+    hipLaunchKernelGGL((k_apply_all<KF, R>), dim3(blocks), dim3(kLanes), 0, s, w, nwork, p, lt, g,
+                       depth, Q);
 }
 
 // Events and pinned counter of the round pipeline, created on first use.
@@ -415,6 +431,7 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
     const int nwork = (int)L.work.size();
     const int64_t max_rounds = 100 * (int64_t)L.nmax + 3;
     const unsigned reserve = cfg.CW >= 4 ? 0u : (unsigned)(kLdsPerCU / (4 * cfg.CW));
+    const int acap = cfg.AW > 0 ? cfg.AW * h->num_cus * 4 : 0;  // apply waves (AW per SIMD)
     // Round r: the chase (chase stream) waits for the apply of round r - 2, which used the
     // same buffers; the apply (apply stream) waits for the chase; the finished count is
     // copied to pinned host memory after the chase.  `first` marks the rounds whose
@@ -435,13 +452,13 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
         if (!Q) {
             // eigenvalues only: nothing to apply (the ordering events stay the same)
         } else if (cfg.KF == 16) {
-            launch_apply<16, 1>(nwork, as, d_work, d_probs, d_lt[b], d_grid[b], d_depth[b], Q);
+            launch_apply<16, 1>(nwork, acap, as, d_work, d_probs, d_lt[b], d_grid[b], d_depth[b], Q);
         } else if (cfg.KF == 8 && cfg.R == 2) {
-            launch_apply<8, 2>(nwork, as, d_work, d_probs, d_lt[b], d_grid[b], d_depth[b], Q);
+            launch_apply<8, 2>(nwork, acap, as, d_work, d_probs, d_lt[b], d_grid[b], d_depth[b], Q);
         } else if (cfg.KF == 8) {
-            launch_apply<8, 1>(nwork, as, d_work, d_probs, d_lt[b], d_grid[b], d_depth[b], Q);
+            launch_apply<8, 1>(nwork, acap, as, d_work, d_probs, d_lt[b], d_grid[b], d_depth[b], Q);
         } else {
-            launch_apply<4, 1>(nwork, as, d_work, d_probs, d_lt[b], d_grid[b], d_depth[b], Q);
+            launch_apply<4, 1>(nwork, acap, as, d_work, d_probs, d_lt[b], d_grid[b], d_depth[b], Q);
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
         return hipEventRecord(applied[b], as);
@@ -471,7 +488,7 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
             (int64_t)d_st, (int64_t)d_depth[0], (int64_t)d_lt[0], (int64_t)d_grid[0],
             (int64_t)d_order,
             (int64_t)d_lt[1], (int64_t)d_grid[1], (int64_t)d_count, (int64_t)pin, B, nwork,
-            L.nmax, cfg.ML, cfg.KF, cfg.R, cfg.CW, (int64_t)cs, (int64_t)as};
+            L.nmax, cfg.ML, cfg.KF, cfg.R, cfg.CW, cfg.AW, (int64_t)cs, (int64_t)as};
         if (!h->batch_graph || h->batch_graph_key != key) {
             if (h->batch_graph) {
                 (void)hipGraphExecDestroy(h->batch_graph);
@@ -673,8 +690,9 @@ int tune_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const int64_t* 
         float ms = 0;
         if ((he = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
         if (std::getenv("SEP_TUNE_VERBOSE"))
-            std::fprintf(stderr, "[sep tune] config %d (ML %d KF %d R %d CW %d): %.2f ms\n", c,
-                         kConfigs[c].ML, kConfigs[c].KF, kConfigs[c].R, kConfigs[c].CW, ms);
+            std::fprintf(stderr, "[sep tune] config %d (ML %d KF %d R %d CW %d AW %d): %.2f ms\n",
+                         c, kConfigs[c].ML, kConfigs[c].KF, kConfigs[c].R, kConfigs[c].CW,
+                         kConfigs[c].AW, ms);
         if (ms < best) {
             best = ms;
             best_i = c;

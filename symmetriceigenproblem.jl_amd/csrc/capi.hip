@@ -240,6 +240,7 @@ int sep_create(sep_handle** out, int device)
     }
     sep_handle* h = new sep_handle();
     h->device = device;
+    h->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     if (const char* ap = std::getenv("SEP_APPLY")) {
         h->force_sgpr_apply = std::strcmp(ap, "sgpr") == 0;
         // Float64 apply geometry (unset: chosen by the number of rows)

@@ -76,6 +76,7 @@ struct ApplyWork {
 
 struct sep_handle {
     int device = 0;
+    int num_cus = 256;  // compute units of the device (hipDeviceProp_t::multiProcessorCount)
     hipStream_t stream = nullptr;        // apply stream (user-visible)
     hipStream_t chase_stream = nullptr;  // chase stream (overlap with apply)
     bool own_stream = true;

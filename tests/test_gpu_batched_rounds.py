@@ -66,7 +66,7 @@ def test_rounds_bitwise_equal_to_oracle(sep, handle, max_layers, fused_depth, ro
         conv, iters, gd, ge, gQ = _solve(sep, handle, ns, d, e, off_d, off_q, max_layers)
     finally:
         handle.set_batched_config(0, 0)
-    ml = max_layers if max_layers > 0 else 32
+    ml = max_layers if max_layers > 0 else 16  # the default block length
     _check_bitwise(ns, d, e, off_d, off_q, ml, conv, iters, gd, ge, gQ)
 
 
