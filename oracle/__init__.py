@@ -248,6 +248,22 @@ def signfix_like(Q, Qref):
     return np.asfortranarray(Q * s[None, :])
 
 
+def fd_directional(d, e, w, G, vd, ve, h: float = 1e-6):
+    """Central difference of L(d, e) = w.lam + <G, Q> (sorted, sign-fixed LAPACK solve)
+    along the direction (vd, ve): (L(x + h v) - L(x - h v)) / 2h, which equals
+    grad L . v up to O(h^2)."""
+    d = np.asarray(d, dtype=np.float64)
+    e = np.asarray(e, dtype=np.float64)
+    _, Q0 = eig_sorted_signfixed(d, e)
+
+    def loss(dd, ee):
+        lam, Q = eig_sorted_signfixed(dd, ee)
+        Q = signfix_like(Q, Q0)
+        return float(w @ lam + np.sum(G * Q))
+
+    return (loss(d + h * vd, e + h * ve) - loss(d - h * vd, e - h * ve)) / (2 * h)
+
+
 def fd_gradient(d, e, w, G, h: float = 1e-6, d_idx=None, e_idx=None):
     """Central differences of L(d, e) = w.lam + <G, Q> (sorted, sign-fixed LAPACK solve)
     on the chosen entries of d and e (all by default)."""

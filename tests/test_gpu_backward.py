@@ -74,6 +74,50 @@ def test_backward_matches_finite_differences(sep, handle, n):
     _check(db.cpu().numpy(), eb.cpu().numpy(), fd_d, fd_e, 1e-6)
 
 
+def test_backward_fd_64_random_entries(sep, handle):
+    """The differentiated-solve check of the north-star workload: central differences
+    (h = 1e-6) on 64 random entries (32 of d, 32 of e) of L = w.lam + <G, Q>, the loss
+    evaluated by LAPACK (sorted, sign-fixed), against the GPU backward at the same point.
+    n = 1024 (each difference is two full eigendecompositions; the n = 4096 problem is
+    checked along a random direction below).  Tolerance 1e-5 of the largest entry
+    (measured agreement 8.5e-7)."""
+    n = 1024
+    d, e = sep.inputs.tridiag_linear(n, seed=1, e_low=0.1)
+    w, G = sep.inputs.backward_loss_weights(n, seed=1)
+    lam, Q = oracle.eig_sorted_signfixed(d, e)
+    db, eb = sep.eig_backward(_dev(lam), _dev_colmajor(sep, Q), _dev(w), _dev_colmajor(sep, G),
+                              handle=handle)
+    db, eb = db.cpu().numpy(), eb.cpu().numpy()
+    rng = np.random.default_rng(1024)
+    di = rng.choice(n, 32, replace=False)
+    ei = rng.choice(n - 1, 32, replace=False)
+    gd, ge = oracle.fd_gradient(d, e, w, G, d_idx=di, e_idx=ei)
+    scale = max(np.abs(db).max(), np.abs(eb).max())
+    err = max(max(abs(db[k] - gd[k]) for k in di), max(abs(eb[k] - ge[k]) for k in ei))
+    assert err <= 1e-5 * scale, (err, scale)
+
+
+def test_backward_fd_directional_n4096(sep, handle):
+    """Full size, n = 4096 (d_i = i, e ~ U[0.1, 1)): grad L . v of the GPU backward against
+    the central difference (h = 1e-6) of L along a random unit direction v over all of
+    (d, e).  Tolerance 1e-5 |grad L| (measured 4e-7)."""
+    n = 4096
+    d, e = sep.inputs.tridiag_linear(n, seed=1, e_low=0.1)
+    w, G = sep.inputs.backward_loss_weights(n)
+    lam, Q = oracle.eig_sorted_signfixed(d, e)
+    db, eb = sep.eig_backward(_dev(lam), _dev_colmajor(sep, Q), _dev(w), _dev_colmajor(sep, G),
+                              handle=handle)
+    db, eb = db.cpu().numpy(), eb.cpu().numpy()
+    r = np.random.default_rng(7)
+    vd, ve = r.standard_normal(n), r.standard_normal(n - 1)
+    nv = np.sqrt(vd @ vd + ve @ ve)
+    vd, ve = vd / nv, ve / nv
+    fd = oracle.fd_directional(d, e, w, G, vd, ve)
+    g = db @ vd + eb @ ve
+    gn = np.sqrt(db @ db + eb @ eb)
+    assert abs(g - fd) <= 1e-5 * gn, (g, fd, gn)
+
+
 def test_backward_full_size_from_gpu_solve(sep, handle):
     """The differentiated-solve workload at full size: n = 4096, d_i = i, e ~ U[0.1, 1),
     eigendecomposition by this library's GPU solver (checked by its residual), then the

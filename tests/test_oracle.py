@@ -374,3 +374,23 @@ def test_backward_dense_formula_eigenvalue_only_closed_form():
     db, eb = oracle.eig_backward_dense(lam, Q, w, np.zeros((n, n)))
     assert np.allclose(db, (Q ** 2) @ w, rtol=0, atol=1e-14)
     assert np.allclose(eb, 2 * (Q[:-1] * Q[1:]) @ w, rtol=0, atol=1e-14)
+
+
+def test_fd_directional_matches_dense_gradient():
+    """oracle.fd_directional (the n = 4096 backward check along a direction) equals the
+    dense-formula gradient projected on the direction, at a size where both are exact
+    to FD accuracy."""
+    n = 60
+    r = np.random.default_rng(60)
+    d, e = np.arange(1.0, n + 1), r.uniform(0.1, 1.0, n - 1)
+    w, G = r.standard_normal(n), np.asfortranarray(r.standard_normal((n, n)))
+    lam, Q = oracle.eig_sorted_signfixed(d, e)
+    gd, ge = oracle.eig_backward_dense(lam, Q, w, G)
+    vd, ve = r.standard_normal(n), r.standard_normal(n - 1)
+    fd = oracle.fd_directional(d, e, w, G, vd, ve)
+    g = gd @ vd + ge @ ve
+    assert abs(g - fd) <= 1e-6 * (abs(g) + np.sqrt(gd @ gd + ge @ ge))
+    # a planted error in the gradient (one entry off by 1e-3) is caught
+    gd2 = gd.copy()
+    gd2[7] += 1e-3
+    assert abs(gd2 @ vd + ge @ ve - fd) > 1e-6 * (abs(g) + np.sqrt(gd @ gd + ge @ ge))
