@@ -452,6 +452,8 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
             const int i0 = __builtin_amdgcn_readfirstlane(i0u);
             const T D = nD, E = nE;
             const int Lt = nL;
+            // this chunk's inputs are complete here (one wait per chunk, not per step)
+            asm volatile("" ::"v"(D), "v"(E));
             {  // prefetch next chunk
                 const int i = i0 + kWave + lane;
                 nD = T(0);
@@ -466,7 +468,8 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
             const int cnt = __builtin_amdgcn_readfirstlane(min(kWave, i_last - i0 + 1));
             // outputs of step i0 + j are kept in lane j and stored once per chunk
             T oR = T(0), oD = T(0), oC = T(0), oS = T(0);
-            for (int j = 0; j < cnt; ++j) {
+            uint64_t jm = 1;  // lane mask of step j
+            for (int j = 0; j < cnt; ++j, jm <<= 1) {
                 const T B22 = rdlane(D, j);
                 const Rot<T> G = givens(p1, bulge);
                 const T c = G.c, s = G.s;
@@ -478,7 +481,7 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                 const T B11pp = __builtin_fma(B11p, c, B12p * s);
                 const T B12pp = __builtin_fma(B11p, -s, B12p * c);
                 const T B22pp = __builtin_fma(B21p, -s, B22p * c);
-                capture_lane(oR, oD, oC, oS, G.r, B11pp, c, s, 1ull << j);
+                capture_lane(oR, oD, oC, oS, G.r, B11pp, c, s, jm);
                 p1 = B12pp;
                 d_cur = B22pp;
                 // B23 = ev[i+1]; the last step of the sweep has none: its prefetch slot
