@@ -186,20 +186,24 @@ __global__ void __launch_bounds__(256) k_coef_table(T* __restrict__ tab,
 // SEP_FMAC names v_fmac_f64_dpp or v_fmac_f32_dpp; it is defined around each use below.
 #define SEP_ROT3(a, b, p, q) /* a += p b; b += q a; a += p b (three shears) */
 
+// Within each phase the layers run from 7 down to 0: only layer 0's rotation reads the
+// column that entered the window at this step (from LDS, or from the Q prefetch), so the
+// other seven rotations of the phase cover that read's latency, which at the first step of
+// an interval follows the barrier.
 #define SEP_SHEAR8                                                                             \
     asm("s_nop 1\n"                                                                            \
-        SEP_DPP("%0", "%16", "%1", 0) SEP_DPP("%2", "%16", "%3", 2) SEP_DPP("%4", "%16", "%5", 4)  \
-        SEP_DPP("%6", "%16", "%7", 6) SEP_DPP("%8", "%16", "%9", 8)                                \
-        SEP_DPP("%10", "%16", "%11", 10) SEP_DPP("%12", "%16", "%13", 12)                        \
-        SEP_DPP("%14", "%16", "%15", 14)                                                       \
-        SEP_DPP("%1", "%16", "%0", 1) SEP_DPP("%3", "%16", "%2", 3) SEP_DPP("%5", "%16", "%4", 5)  \
-        SEP_DPP("%7", "%16", "%6", 7) SEP_DPP("%9", "%16", "%8", 9)                                \
-        SEP_DPP("%11", "%16", "%10", 11) SEP_DPP("%13", "%16", "%12", 13)                        \
-        SEP_DPP("%15", "%16", "%14", 15)                                                       \
-        SEP_DPP("%0", "%16", "%1", 0) SEP_DPP("%2", "%16", "%3", 2) SEP_DPP("%4", "%16", "%5", 4)  \
-        SEP_DPP("%6", "%16", "%7", 6) SEP_DPP("%8", "%16", "%9", 8)                                \
-        SEP_DPP("%10", "%16", "%11", 10) SEP_DPP("%12", "%16", "%13", 12)                        \
-        SEP_DPP("%14", "%16", "%15", 14)                                                       \
+        SEP_DPP("%14", "%16", "%15", 14) SEP_DPP("%12", "%16", "%13", 12)                        \
+        SEP_DPP("%10", "%16", "%11", 10) SEP_DPP("%8", "%16", "%9", 8)                           \
+        SEP_DPP("%6", "%16", "%7", 6) SEP_DPP("%4", "%16", "%5", 4) SEP_DPP("%2", "%16", "%3", 2)  \
+        SEP_DPP("%0", "%16", "%1", 0)                                                          \
+        SEP_DPP("%15", "%16", "%14", 15) SEP_DPP("%13", "%16", "%12", 13)                        \
+        SEP_DPP("%11", "%16", "%10", 11) SEP_DPP("%9", "%16", "%8", 9)                           \
+        SEP_DPP("%7", "%16", "%6", 7) SEP_DPP("%5", "%16", "%4", 5) SEP_DPP("%3", "%16", "%2", 3)  \
+        SEP_DPP("%1", "%16", "%0", 1)                                                          \
+        SEP_DPP("%14", "%16", "%15", 14) SEP_DPP("%12", "%16", "%13", 12)                        \
+        SEP_DPP("%10", "%16", "%11", 10) SEP_DPP("%8", "%16", "%9", 8)                           \
+        SEP_DPP("%6", "%16", "%7", 6) SEP_DPP("%4", "%16", "%5", 4) SEP_DPP("%2", "%16", "%3", 2)  \
+        SEP_DPP("%0", "%16", "%1", 0)                                                          \
         : "+v"(SLOT(0, 0)), "+v"(SLOT(0, 1)), "+v"(SLOT(1, 0)), "+v"(SLOT(1, 1)),              \
           "+v"(SLOT(2, 0)), "+v"(SLOT(2, 1)), "+v"(SLOT(3, 0)), "+v"(SLOT(3, 1)),              \
           "+v"(SLOT(4, 0)), "+v"(SLOT(4, 1)), "+v"(SLOT(5, 0)), "+v"(SLOT(5, 1)),              \
