@@ -160,6 +160,22 @@ int sep_eig_backward_batched_f64_dev(sep_handle *h, int64_t B, const int64_t *n,
                                      const double *lam, const double *Q, const double *lam_bar,
                                      const double *Q_bar, double *d_bar, double *e_bar);
 
+/* ---- Dense symmetric -> tridiagonal (no counterpart in the reference) ----------------
+ * The reference's qr_algorithm! starts from a SymTridiagonal (README.md:3); these two
+ * entries put the reduction in front of it for a dense symmetric A, with LAPACK's
+ * conventions (dsytrd / dorgtr, uplo = 'L'), so that
+ *     sep_sytrd_f64_dev -> sep_orgtr_f64_dev -> sep_qr_algorithm_f64_dev(d, e, Q)
+ * leaves the eigenvalues in d and the eigenvectors of the dense A in Q.
+ * sytrd: only the lower triangle of A (n x n, column-major, ld lda) is referenced; on
+ * return d (n) and e (n-1) hold T, tau (n-1) and the strictly lower part of A below the
+ * subdiagonal hold the reflectors H(i) = I - tau_i v_i v_i^T (v_i[i+1] = 1), and A[i+1, i]
+ * = e[i].  orgtr: Q (n x n, ld ldq) <- H(0) H(1) ... H(n-2) from that A and tau.
+ * All pointers are device pointers on the handle's stream; n <= 65536. */
+int sep_sytrd_f64_dev(sep_handle *h, int64_t n, double *A, int64_t lda, double *d, double *e,
+                      double *tau);
+int sep_orgtr_f64_dev(sep_handle *h, int64_t n, const double *A, int64_t lda, const double *tau,
+                      double *Q, int64_t ldq);
+
 #ifdef __cplusplus
 }
 #endif

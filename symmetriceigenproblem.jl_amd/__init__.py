@@ -55,6 +55,7 @@ EXPORTS = (
     "sep_qr_batched_f64_dev", "sep_tune_batched_f64", "sep_set_batched_config",
     "sep_set_odd_order",
     "sep_eig_backward_f64_dev", "sep_eig_backward_batched_f64_dev",
+    "sep_sytrd_f64_dev", "sep_orgtr_f64_dev",
 )
 
 _lib: Optional[C.CDLL] = None
@@ -109,6 +110,8 @@ def _declare(L: C.CDLL) -> None:
     L.sep_set_odd_order.argtypes = [_P, C.c_int32]
     L.sep_eig_backward_f64_dev.argtypes = [_P, _I64, _P, _P, _I64, _P, _P, _P, _P]
     L.sep_eig_backward_batched_f64_dev.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P]
+    L.sep_sytrd_f64_dev.argtypes = [_P, _I64, _P, _I64, _P, _P, _P]
+    L.sep_orgtr_f64_dev.argtypes = [_P, _I64, _P, _I64, _P, _P, _I64]
     for name in EXPORTS:
         if name not in ("sep_last_error", "sep_version"):
             getattr(L, name).restype = C.c_int
@@ -433,3 +436,58 @@ def eig_backward_batched(ns, off_d, off_q, lam, Q, lam_bar, Q_bar, handle: Optio
         lam.data_ptr(), Q.data_ptr(), lam_bar.data_ptr(), Q_bar.data_ptr(),
         d_bar.data_ptr(), e_bar.data_ptr()))
     return d_bar, e_bar
+
+
+# ---------------------------------------------------------------- dense input (no reference
+# counterpart: the reference starts from a SymTridiagonal)
+
+def sytrd_(A, handle: Optional[Handle] = None):
+    """Reduce a dense symmetric A (Float64, column-major, lower triangle referenced) to
+    tridiagonal T = Q^T A Q in place, LAPACK dsytrd conventions (uplo = 'L').
+
+    Returns (d, e, tau); A then holds e on its subdiagonal and the reflectors below it.
+    A torch device tensor runs in place on the device (d, e, tau are device tensors); a
+    Fortran-ordered numpy array is copied to the device and back (numpy results)."""
+    import torch
+    h = handle or default_handle()
+    host = not _is_torch(A)
+    Ad = to_device_colmajor(A) if host else A
+    m, n, lda = _torch_colmajor(Ad)
+    if m != n or Ad.dtype != torch.float64:
+        raise ValueError("A must be square Float64")
+    d = torch.empty(max(n, 1), dtype=torch.float64, device=Ad.device)
+    e = torch.empty(max(n - 1, 1), dtype=torch.float64, device=Ad.device)
+    tau = torch.empty(max(n - 1, 1), dtype=torch.float64, device=Ad.device)
+    _check(lib().sep_sytrd_f64_dev(h.ptr, n, Ad.data_ptr(), lda, d.data_ptr(), e.data_ptr(),
+                                   tau.data_ptr()))
+    d, e, tau = d[:n], e[: max(n - 1, 0)], tau[: max(n - 1, 0)]
+    if host:
+        A[...] = to_host_colmajor(Ad)
+        return d.cpu().numpy(), e.cpu().numpy(), tau.cpu().numpy()
+    return d, e, tau
+
+
+def orgtr(A, tau, handle: Optional[Handle] = None):
+    """The orthogonal factor Q = H(0) ... H(n-2) of sytrd_'s output (LAPACK dorgtr, 'L')."""
+    import torch
+    h = handle or default_handle()
+    host = not _is_torch(A)
+    Ad = to_device_colmajor(A) if host else A
+    td = torch.from_numpy(np.ascontiguousarray(tau)).to(Ad.device) if host else tau
+    m, n, lda = _torch_colmajor(Ad)
+    Q = torch.empty((n, n), dtype=torch.float64, device=Ad.device).t()
+    _check(lib().sep_orgtr_f64_dev(h.ptr, n, Ad.data_ptr(), lda,
+                                   td.data_ptr() if n > 1 else None, Q.data_ptr(), max(n, 1)))
+    return to_host_colmajor(Q) if host else Q
+
+
+def eigh_dense_(A, max_layers: int = 128, handle: Optional[Handle] = None):
+    """Eigen-decomposition of a dense symmetric Float64 A (lower triangle): sytrd_, orgtr,
+    then qr_algorithm_ on T starting from that Q.  Returns (lam, Q, converged, iter) with
+    lam in the solver's deflation order and A Q = Q diag(lam); A is overwritten."""
+    d, e, tau = sytrd_(A, handle=handle)
+    Q = orgtr(A, tau, handle=handle)
+    H = SymTridiagonal(d, e)
+    _, conv, it = qr_algorithm_(H, Q, max_layers, handle=handle)
+    return H.dv, Q, conv, it
+

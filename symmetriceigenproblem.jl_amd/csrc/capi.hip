@@ -290,6 +290,8 @@ int sep_destroy(sep_handle* h)
     h->bwd_meta.release();
     h->bwd_M.release();
     h->bwd_part.release();
+    h->trd_vec.release();
+    h->trd_part.release();
     for (auto e : h->tev) (void)hipEventDestroy(e);
     if (h->batch_graph) (void)hipGraphExecDestroy(h->batch_graph);
     for (auto bs : h->batch_stream)
@@ -495,6 +497,28 @@ int sep_eig_backward_batched_f64_dev(sep_handle* h, int64_t B, const int64_t* n,
     }
     SEP_HIP(hipSetDevice(h->device));
     return eig_backward_batched_f64(h, B, n, off_d, off_q, lam, Q, lam_bar, Q_bar, d_bar, e_bar);
+}
+
+int sep_sytrd_f64_dev(sep_handle* h, int64_t n, double* A, int64_t lda, double* d, double* e,
+                      double* tau)
+{
+    if (!h) {
+        set_error("sep_sytrd: null handle");
+        return SEP_ERR_ARG;
+    }
+    SEP_HIP(hipSetDevice(h->device));
+    return sytrd_f64(h, n, A, lda, d, e, tau);
+}
+
+int sep_orgtr_f64_dev(sep_handle* h, int64_t n, const double* A, int64_t lda, const double* tau,
+                      double* Q, int64_t ldq)
+{
+    if (!h) {
+        set_error("sep_orgtr: null handle");
+        return SEP_ERR_ARG;
+    }
+    SEP_HIP(hipSetDevice(h->device));
+    return orgtr_f64(h, n, A, lda, tau, Q, ldq);
 }
 
 }  // extern "C"

@@ -88,6 +88,8 @@ struct sep_handle {
     sep::DevBuf batch_meta, batch_scratch;
     // backward: problem table (device + the host copy it was uploaded from), M, partials
     sep::DevBuf bwd_meta, bwd_M, bwd_part;
+    // dense -> tridiagonal: reflector / update vectors and per-tile partial sums
+    sep::DevBuf trd_vec, trd_part;
     std::vector<char> bwd_meta_host;
     void* bwd_meta_dev = nullptr;
     // batched solver: host copy of the per-problem state words, autotuner choice
@@ -158,5 +160,11 @@ int eig_backward_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const i
                              const int64_t* off_q, const double* lam, const double* Q,
                              const double* lam_bar, const double* Q_bar, double* d_bar,
                              double* e_bar);
+
+// tridiag.hip
+int sytrd_f64(sep_handle* h, int64_t n, double* A, int64_t lda, double* d, double* e,
+              double* tau);
+int orgtr_f64(sep_handle* h, int64_t n, const double* A, int64_t lda, const double* tau,
+              double* Q, int64_t ldq);
 
 }  // namespace sep

@@ -1,0 +1,384 @@
+// tridiag.hip -- dense symmetric -> tridiagonal reduction and the explicit orthogonal
+// factor, the step before qr_algorithm! when the input is a dense symmetric matrix.
+//
+// The reference starts from a SymTridiagonal (its README states the scope); this file adds
+// the reduction in front of it with LAPACK's storage conventions, so the results can be
+// compared with dsytrd / dorgtr (uplo = 'L'):
+//   sytrd: A = Q T Q^T, Q = H(0) H(1) ... H(n-2), H(i) = I - tau_i v_i v_i^T with
+//          v_i[i+1] = 1 and v_i[i+2 .. n-1] stored in A[i+2 .., i]; d = diag(T),
+//          e = subdiag(T), also left in A[i+1, i].  Only the lower triangle is referenced.
+//   orgtr: the n x n matrix Q from those reflectors.
+// Started from Q, qr_algorithm!(T, Q) accumulates the tridiagonal eigenvectors onto it,
+// so its output columns are eigenvectors of the dense A.
+//
+// Reduction, column i (dsytd2's unblocked order): reflector of the updated column i, then
+// y = tau A22 v, w = y - (tau/2)(y.v) v and the rank-2 update A22 -= v w^T + w v^T.  The
+// update is applied lazily: the pass that computes y for column i first applies column
+// i-1's pending update to each element it reads, so every column costs one read and one
+// write of the trailing lower triangle (the HBM bound: 8 n^3 / 3 bytes in all).  Per column:
+//   k_trd_tiles   one 64 x 64 lower tile per workgroup: update, store, and the tile's
+//                 contributions A_IJ v_J (to row block I) and A_IJ^T v_I (to row block J)
+//                 written to a partial slot of their own (slot = the other tile index);
+//   k_trd_reduce  sums the slots in a fixed order (deterministic, no atomics);
+//   k_trd_vec     one workgroup: w of the previous column, its update applied to column i,
+//                 d[i], and the reflector of column i (dlarfg: beta = -sign(alpha)
+//                 hypot(alpha, |x|), tau = (beta - alpha) / beta, v = x / (alpha - beta)).
+// The explicit factor is accumulated backwards (Q = H(0) (H(1) (... H(n-2)))), again with
+// each reflector's rank-1 update applied in the pass that forms the next reflector's
+// column dots z = v^T Q.
+#include "sep_internal.hpp"
+
+namespace sep {
+namespace {
+
+constexpr int kTile = 64;
+constexpr int kThreads = 256;  // 4 waves; wave q owns 16 columns of the tile
+constexpr int kVecThreads = 1024;
+
+// Tile (I, J), I >= J, of the lower triangle from its linear index t = I (I + 1) / 2 + J.
+__device__ __forceinline__ void lower_tile(int t, int& I, int& J)
+{
+    int r = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((r + 1) * (r + 2) / 2 <= t) ++r;
+    while (r * (r + 1) / 2 > t) --r;
+    I = r;
+    J = t - r * (r + 1) / 2;
+}
+
+// Column i's trailing block A[s.., s..], s = i + 1, m = n - s.  vp / wp: the pending update
+// of column i - 1 (absolute row indices; has_pending = 0 for i = 0); vn: column i's
+// reflector.  part[slot * m + row] receives the tile contributions to y = A22 vn.
+__global__ void __launch_bounds__(kThreads)
+    k_trd_tiles(double* __restrict__ A, int64_t lda, int s, int m, const double* __restrict__ vp,
+                const double* __restrict__ wp, int has_pending, const double* __restrict__ vn,
+                double* __restrict__ part)
+{
+    __shared__ double prod[kTile][kTile + 1];  // [column][row] products a * vn[row]
+    __shared__ double rows[4][kTile];
+    int I, J;
+    lower_tile(blockIdx.x, I, J);
+    const int lane = threadIdx.x % kTile, q = threadIdx.x / kTile;
+    const int row = I * kTile + lane;
+    const bool rin = row < m;
+    const double vr = rin ? vn[s + row] : 0.0;
+    const double vpr = (rin && has_pending) ? vp[s + row] : 0.0;
+    const double wpr = (rin && has_pending) ? wp[s + row] : 0.0;
+    double* __restrict__ colbase = A + (s + row);
+    double a[16];
+    int col0 = J * kTile + q * 16;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        const int col = col0 + c;
+        const bool ok = rin && col < m && (I > J || row >= col);
+        a[c] = ok ? colbase[(int64_t)(s + col) * lda] : 0.0;
+    }
+    double rs = 0.0;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        const int col = col0 + c;
+        const bool ok = rin && col < m && (I > J || row >= col);
+        if (ok && has_pending) {
+            a[c] -= vpr * wp[s + col] + wpr * vp[s + col];
+            colbase[(int64_t)(s + col) * lda] = a[c];
+        }
+        const double vc = col < m ? vn[s + col] : 0.0;
+        rs += a[c] * vc;
+        // transposed contribution: strictly-lower elements only (the diagonal is counted
+        // once, in the row sum)
+        prod[q * 16 + c][lane] = (I > J || row > col) ? a[c] * vr : 0.0;
+    }
+    rows[q][lane] = rs;
+    __syncthreads();
+    // column sums: thread (cc, q) adds rows q*16 .. q*16+15 of column cc
+    const int cc = threadIdx.x % kTile;
+    double cs = 0.0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) cs += prod[cc][q * 16 + r];
+    __syncthreads();
+    prod[cc][q] = cs;  // reuse: column cc's four partials in prod[cc][0..3]
+    __syncthreads();
+    if (threadIdx.x < kTile) {
+        const int x = threadIdx.x;
+        const double rsum = ((rows[0][x] + rows[1][x]) + rows[2][x]) + rows[3][x];
+        const double csum = ((prod[x][0] + prod[x][1]) + prod[x][2]) + prod[x][3];
+        if (I == J) {
+            if (I * kTile + x < m) part[(int64_t)I * m + I * kTile + x] = rsum + csum;
+        } else {
+            if (I * kTile + x < m) part[(int64_t)J * m + I * kTile + x] = rsum;
+            if (J * kTile + x < m) part[(int64_t)I * m + J * kTile + x] = csum;
+        }
+    }
+}
+
+// y[s + r] = sum over slots of part[slot * m + r] (fixed order).
+__global__ void __launch_bounds__(256)
+    k_trd_reduce(const double* __restrict__ part, int nslots, int s, int m, double* __restrict__ y)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m) return;
+    double acc = 0.0;
+    for (int t = 0; t < nslots; ++t) acc += part[(int64_t)t * m + r];
+    y[s + r] = acc;
+}
+
+__device__ double block_sum(double v, double* red)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int k = 0; k < (int)(blockDim.x / 64); ++k) t += red[k];
+    return t;
+}
+__device__ double block_max(double v, double* red)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int k = 0; k < (int)(blockDim.x / 64); ++k) t = fmax(t, red[k]);
+    return t;
+}
+
+// Column i: finish the previous column's w (from y = A22 v of column i - 1), apply its
+// update to column i, write d[i] and generate column i's reflector into A, vn, e, tau.
+__global__ void __launch_bounds__(kVecThreads)
+    k_trd_vec(double* __restrict__ A, int64_t lda, int n, int i, const double* __restrict__ vp,
+              double* __restrict__ wp, const double* __restrict__ y, double* __restrict__ vn,
+              double* __restrict__ d, double* __restrict__ e, double* __restrict__ tau)
+{
+    __shared__ double red[kVecThreads / 64];
+    double* __restrict__ col = A + (int64_t)i * lda;
+    if (i > 0) {
+        // previous column p = i - 1, rows [i, n): x = tau_p y, w = x - (tau_p/2)(x.v) v
+        const double tp = tau[i - 1];
+        double dot = 0.0;
+        for (int r = i + threadIdx.x; r < n; r += blockDim.x) dot += (tp * y[r]) * vp[r];
+        dot = block_sum(dot, red);
+        const double alpha = -0.5 * tp * dot;
+        for (int r = i + threadIdx.x; r < n; r += blockDim.x) wp[r] = tp * y[r] + alpha * vp[r];
+        __syncthreads();
+        const double vi = vp[i], wi = wp[i];
+        for (int r = i + threadIdx.x; r < n; r += blockDim.x)
+            col[r] -= vp[r] * wi + wp[r] * vi;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) d[i] = col[i];
+    if (i >= n - 1) return;
+    // dlarfg on alpha = A[i+1, i], x = A[i+2 .., i]
+    const double alpha = col[i + 1];
+    double mx = 0.0;
+    for (int r = i + 2 + threadIdx.x; r < n; r += blockDim.x) mx = fmax(mx, fabs(col[r]));
+    mx = block_max(mx, red);
+    double ss = 0.0;
+    if (mx > 0.0) {
+        const double inv = 1.0 / mx;
+        for (int r = i + 2 + threadIdx.x; r < n; r += blockDim.x) {
+            const double t = col[r] * inv;
+            ss += t * t;
+        }
+    }
+    ss = block_sum(ss, red);
+    const double xnorm = mx * sqrt(ss);
+    if (xnorm == 0.0) {
+        for (int r = i + 1 + threadIdx.x; r < n; r += blockDim.x) vn[r] = r == i + 1 ? 1.0 : 0.0;
+        if (threadIdx.x == 0) {
+            e[i] = alpha;
+            tau[i] = 0.0;
+        }
+        return;
+    }
+    const double beta = -copysign(hypot(alpha, xnorm), alpha);
+    const double t = (beta - alpha) / beta;
+    const double sc = 1.0 / (alpha - beta);
+    __syncthreads();  // every thread has read col[i + 1] and x before they change
+    for (int r = i + 2 + threadIdx.x; r < n; r += blockDim.x) {
+        const double v = col[r] * sc;
+        col[r] = v;
+        vn[r] = v;
+    }
+    if (threadIdx.x == 0) {
+        vn[i + 1] = 1.0;
+        col[i + 1] = beta;
+        e[i] = beta;
+        tau[i] = t;
+    }
+}
+
+// Q <- I (n x n, ld ldq).
+__global__ void k_eye(double* __restrict__ Q, int64_t ldq, int n)
+{
+    const int64_t tot = (int64_t)n * n;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = q / n, r = q % n;
+        Q[r + c * ldq] = r == c ? 1.0 : 0.0;
+    }
+}
+
+// Reflector j's column of A as v_j[r] (r >= j + 1; v_j[j + 1] = 1).
+__device__ __forceinline__ double refl(const double* __restrict__ A, int64_t lda, int j, int r)
+{
+    return r == j + 1 ? 1.0 : A[r + (int64_t)j * lda];
+}
+
+// Backward accumulation, step for reflector i (i = -1: the final pass).  Region
+// [s, n)^2 with s = i + 2: applies reflector i+1's pending update
+// Q[r, c] -= tau_{i+1} v_{i+1}[r] z[c] and, for i >= 0, writes the column partials of
+// v_i^T Q over the region's rows (one slot per row tile) to part.
+__global__ void __launch_bounds__(kThreads)
+    k_org_tiles(double* __restrict__ Q, int64_t ldq, const double* __restrict__ A, int64_t lda,
+                const double* __restrict__ tau, int i, int s, int m, int ntc,
+                const double* __restrict__ zp, double* __restrict__ part)
+{
+    __shared__ double prod[kTile][kTile + 1];
+    const int I = blockIdx.x / ntc, J = blockIdx.x % ntc;
+    const int lane = threadIdx.x % kTile, q = threadIdx.x / kTile;
+    const int row = I * kTile + lane;
+    const bool rin = row < m;
+    const int p = i + 1;  // pending reflector
+    const double tp = tau[p];
+    const double vpr = rin ? refl(A, lda, p, s + row) : 0.0;
+    const double vnr = (rin && i >= 0) ? refl(A, lda, i, s + row) : 0.0;
+    double* __restrict__ base = Q + (s + row);
+    const int col0 = J * kTile + q * 16;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        const int col = col0 + c;
+        double a = 0.0;
+        if (rin && col < m) {
+            double* pq = base + (int64_t)(s + col) * ldq;
+            a = *pq - tp * vpr * zp[s + col];
+            *pq = a;
+        }
+        prod[q * 16 + c][lane] = a * vnr;
+    }
+    if (i < 0) return;
+    __syncthreads();
+    const int cc = threadIdx.x % kTile;
+    double cs = 0.0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) cs += prod[cc][q * 16 + r];
+    __syncthreads();
+    prod[cc][q] = cs;
+    __syncthreads();
+    if (threadIdx.x < kTile) {
+        const int x = threadIdx.x, col = J * kTile + x;
+        if (col < m)
+            part[(int64_t)I * m + col] = ((prod[x][0] + prod[x][1]) + prod[x][2]) + prod[x][3];
+    }
+}
+
+// z_i over columns [i+1, n): z[i+1] = 1 (row i+1 of Q is still e_{i+1}), z[c] = sum of the
+// row-tile partials for c >= i+2.
+__global__ void __launch_bounds__(256)
+    k_org_reduce(const double* __restrict__ part, int nslots, int i, int s, int m,
+                 double* __restrict__ z)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r == 0) z[i + 1] = 1.0;
+    if (r >= m) return;
+    double acc = 0.0;
+    for (int t = 0; t < nslots; ++t) acc += part[(int64_t)t * m + r];
+    z[s + r] = acc;
+}
+
+int check_args(const char* who, int64_t n, int64_t lda, const void* A)
+{
+    if (n < 0 || n > 65536 || lda < std::max<int64_t>(n, 1)) {
+        set_error("%s: invalid n=%lld ld=%lld", who, (long long)n, (long long)lda);
+        return SEP_ERR_ARG;
+    }
+    if (n > 0 && !A) {
+        set_error("%s: null device pointer", who);
+        return SEP_ERR_ARG;
+    }
+    return SEP_OK;
+}
+
+}  // namespace
+
+int sytrd_f64(sep_handle* h, int64_t n64, double* A, int64_t lda, double* d, double* e,
+              double* tau)
+{
+    if (int rc = check_args("sep_sytrd", n64, lda, A)) return rc;
+    if (n64 == 0) return SEP_OK;
+    if (!d || (n64 > 1 && (!e || !tau))) {
+        set_error("sep_sytrd: null device pointer");
+        return SEP_ERR_ARG;
+    }
+    const int n = (int)n64;
+    hipStream_t st = h->stream;
+    if (n == 1) {
+        SEP_HIP(hipMemcpyAsync(d, A, sizeof(double), hipMemcpyDeviceToDevice, st));
+        return SEP_OK;
+    }
+    const int ntmax = (n - 1 + kTile - 1) / kTile;
+    // vectors: v[2][n], w[n], y[n]
+    if (int rc = h->trd_vec.reserve(sizeof(double) * 4 * (size_t)n)) return rc;
+    if (int rc = h->trd_part.reserve(sizeof(double) * (size_t)ntmax * (size_t)(n - 1))) return rc;
+    double* v[2] = {h->trd_vec.as<double>(), h->trd_vec.as<double>() + n};
+    double* w = h->trd_vec.as<double>() + 2 * (size_t)n;
+    double* y = h->trd_vec.as<double>() + 3 * (size_t)n;
+    double* part = h->trd_part.as<double>();
+    for (int i = 0; i < n; ++i) {
+        // column i: previous column's w and update, d[i], reflector i (into v[i & 1])
+        hipLaunchKernelGGL(k_trd_vec, dim3(1), dim3(kVecThreads), 0, st, A, lda, n, i,
+                           v[(i + 1) & 1], w, y, v[i & 1], d, e, tau);
+        SEP_CHECK_LAUNCH();
+        if (i >= n - 1) break;
+        const int s = i + 1, m = n - s, nt = (m + kTile - 1) / kTile;
+        hipLaunchKernelGGL(k_trd_tiles, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(kThreads), 0,
+                           st, A, lda, s, m, v[(i + 1) & 1], w, i > 0 ? 1 : 0, v[i & 1], part);
+        SEP_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_trd_reduce, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st,
+                           part, nt, s, m, y);
+        SEP_CHECK_LAUNCH();
+    }
+    return SEP_OK;
+}
+
+int orgtr_f64(sep_handle* h, int64_t n64, const double* A, int64_t lda, const double* tau,
+              double* Q, int64_t ldq)
+{
+    if (int rc = check_args("sep_orgtr", n64, lda, A)) return rc;
+    if (int rc = check_args("sep_orgtr", n64, ldq, Q)) return rc;
+    if (n64 > 1 && !tau) {
+        set_error("sep_orgtr: null device pointer");
+        return SEP_ERR_ARG;
+    }
+    if (n64 == 0) return SEP_OK;
+    const int n = (int)n64;
+    hipStream_t st = h->stream;
+    hipLaunchKernelGGL(k_eye, dim3((unsigned)std::min<int64_t>(4096, ((int64_t)n * n + 255) / 256)),
+                       dim3(256), 0, st, Q, ldq, n);
+    SEP_CHECK_LAUNCH();
+    if (n == 1) return SEP_OK;
+    const int ntmax = (n - 1 + kTile - 1) / kTile;
+    if (int rc = h->trd_vec.reserve(sizeof(double) * 4 * (size_t)n)) return rc;
+    if (int rc = h->trd_part.reserve(sizeof(double) * (size_t)ntmax * (size_t)(n - 1))) return rc;
+    double* z[2] = {h->trd_vec.as<double>(), h->trd_vec.as<double>() + n};
+    double* part = h->trd_part.as<double>();
+    // reflector n-2 first: its column dots over rows [n-1, n) are just z[n-1] = 1
+    hipLaunchKernelGGL(k_org_reduce, dim3(1), dim3(256), 0, st, part, 0, n - 2, n, 0,
+                       z[(n - 2) & 1]);
+    SEP_CHECK_LAUNCH();
+    for (int i = n - 3; i >= -1; --i) {
+        // region [i+2, n): pending reflector i+1 (dots in z[(i+1) & 1]); dots of reflector i
+        const int s = i + 2, m = n - s, nt = (m + kTile - 1) / kTile;
+        hipLaunchKernelGGL(k_org_tiles, dim3((unsigned)(nt * nt)), dim3(kThreads), 0, st, Q,
+                           ldq, A, lda, tau, i, s, m, nt, z[(i + 1) & 1], part);
+        SEP_CHECK_LAUNCH();
+        if (i < 0) break;
+        hipLaunchKernelGGL(k_org_reduce, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st,
+                           part, nt, i, s, m, z[i & 1]);
+        SEP_CHECK_LAUNCH();
+    }
+    return SEP_OK;
+}
+
+}  // namespace sep
