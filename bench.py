@@ -15,8 +15,9 @@ Also reported in the same line:
 * extra: the other apply shapes (n = 2000 Float64/Float32, n = 16384 Float32), the plain
   solve of the README example (n = 2000) beside the oracle's CPU solve, the single large
   solve (n = 16384, max_layers = 64), the
-  differentiated-solve workload (n = 4096: forward solve, backward, and their ratio), and
-  the batched workload (4096 problems: solves/s against the per-problem GPU loop).
+  differentiated-solve workload (n = 4096: forward solve, backward, and their ratio), the
+  dense-input reduction (n = 4096: sytrd against its HBM floor, orgtr), and the batched
+  workload (4096 problems: solves/s against the per-problem GPU loop).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
                       [--calibrate-copy-gib G]
@@ -99,7 +100,7 @@ def time_apply(sep, h, torch, n, k, dtype, steps=5):
 
 def extras(sep, h, torch, np, cpu=True, large=True):
     """Secondary figures: other apply shapes, config-1 solve, the differentiated solve, the
-    batched solve."""
+    dense-input reduction (sytrd / orgtr), the batched solve."""
     sys.path.insert(0, ROOT)
     ex = {}
     for n, dt, name in ((2000, torch.float64, "apply_n2000_f64"),
@@ -181,6 +182,33 @@ def extras(sep, h, torch, np, cpu=True, large=True):
     ex["ad_n4096_backward_tflops"] = round(4.0 * n ** 3 / bwd / 1e12, 2)
     log(f"[bench] AD n=4096: forward {fwd:.3f} s, backward {bwd * 1e3:.2f} ms "
         f"({ex['ad_n4096_backward_tflops']} TFLOP/s on 4 n^3), ratio {bwd / fwd:.5f}")
+
+    # dense symmetric input (no reference counterpart): reduction to tridiagonal and the
+    # explicit orthogonal factor at n = 4096, A = (X + X^T) / 2 with X ~ N(0, 1)
+    n = 4096
+    r = np.random.default_rng(3)
+    X = r.standard_normal((n, n))
+    Ah = np.asfortranarray((X + X.T) / 2)
+    t_s, t_o = float("inf"), float("inf")
+    for _ in range(2):
+        Ad = sep.to_device_colmajor(Ah)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, _, tau = sep.sytrd_(Ad, handle=h)
+        torch.cuda.synchronize()
+        t_s = min(t_s, time.perf_counter() - t)
+        t = time.perf_counter()
+        sep.orgtr(Ad, tau, handle=h)
+        torch.cuda.synchronize()
+        t_o = min(t_o, time.perf_counter() - t)
+    # one read and one write of the trailing lower triangle per column: 8 n^3 / 3 bytes
+    floor_s = 8.0 * n ** 3 / 3 / 8e12
+    ex["sytrd_n4096_s"] = round(t_s, 4)
+    ex["sytrd_n4096_hbm_floor_frac"] = round(floor_s / t_s, 3)
+    ex["orgtr_n4096_s"] = round(t_o, 4)
+    ex["orgtr_n4096_tflops"] = round(4.0 * n ** 3 / 3 / t_o / 1e12, 2)
+    log(f"[bench] dense n=4096: sytrd {t_s * 1e3:.1f} ms ({100 * floor_s / t_s:.0f}% of its "
+        f"HBM floor), orgtr {t_o * 1e3:.1f} ms ({ex['orgtr_n4096_tflops']} TFLOP/s on 4n^3/3)")
 
     # config 4: 4096 uneven problems, even n in [32, 512], d_i = i + N(0, 0.1), e ~ U[0,1)
     B = 4096

@@ -55,8 +55,8 @@ def build(asm: bool = False, jobs: int | None = None) -> str:
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, asm), SOURCES))
     tmp = LIB + ".tmp"
-    subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs],
-                   check=True)
+    subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs,
+                    "-L" + os.path.join(ROCM, "lib"), "-lrocblas"], check=True)
     os.replace(tmp, LIB)
     return LIB
 

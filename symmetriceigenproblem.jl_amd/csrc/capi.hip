@@ -292,6 +292,8 @@ int sep_destroy(sep_handle* h)
     h->bwd_part.release();
     h->trd_vec.release();
     h->trd_part.release();
+    h->trd_blk.release();
+    release_blas(h);
     for (auto e : h->tev) (void)hipEventDestroy(e);
     if (h->batch_graph) (void)hipGraphExecDestroy(h->batch_graph);
     for (auto bs : h->batch_stream)

@@ -89,7 +89,8 @@ struct sep_handle {
     // backward: problem table (device + the host copy it was uploaded from), M, partials
     sep::DevBuf bwd_meta, bwd_M, bwd_part;
     // dense -> tridiagonal: reflector / update vectors and per-tile partial sums
-    sep::DevBuf trd_vec, trd_part;
+    sep::DevBuf trd_vec, trd_part, trd_blk;
+    void* blas = nullptr;  // rocblas_handle of the blocked orthogonal-factor GEMMs (lazy)
     std::vector<char> bwd_meta_host;
     void* bwd_meta_dev = nullptr;
     // batched solver: host copy of the per-problem state words, autotuner choice
@@ -166,5 +167,6 @@ int sytrd_f64(sep_handle* h, int64_t n, double* A, int64_t lda, double* d, doubl
               double* tau);
 int orgtr_f64(sep_handle* h, int64_t n, const double* A, int64_t lda, const double* tau,
               double* Q, int64_t ldq);
+void release_blas(sep_handle* h);
 
 }  // namespace sep

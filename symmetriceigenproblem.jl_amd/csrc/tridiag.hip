@@ -26,6 +26,8 @@
 // The explicit factor is accumulated backwards (Q = H(0) (H(1) (... H(n-2)))), again with
 // each reflector's rank-1 update applied in the pass that forms the next reflector's
 // column dots z = v^T Q.
+#include <rocblas/rocblas.h>
+
 #include "sep_internal.hpp"
 
 namespace sep {
@@ -117,6 +119,7 @@ __global__ void __launch_bounds__(256)
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= m) return;
     double acc = 0.0;
+#pragma unroll 8
     for (int t = 0; t < nslots; ++t) acc += part[(int64_t)t * m + r];
     y[s + r] = acc;
 }
@@ -144,23 +147,140 @@ __device__ double block_max(double v, double* red)
     return t;
 }
 
+constexpr int kPer = 8;            // rows per thread that k_trd_vec keeps in registers
+constexpr int kFuseSlots = 16;     // at most this many partial slots: summed in k_trd_vec
+
+// y[r] of the previous column: the reduce kernel's output, or (part != nullptr) the sum of
+// the tile partials in the same fixed slot order.
+__device__ __forceinline__ double y_at(const double* __restrict__ y,
+                                       const double* __restrict__ part, int nslots, int m,
+                                       int s, int r)
+{
+    if (!part) return y[r];
+    double acc = 0.0;
+    for (int t = 0; t < nslots; ++t) acc += part[(int64_t)t * m + (r - s)];
+    return acc;
+}
+
 // Column i: finish the previous column's w (from y = A22 v of column i - 1), apply its
 // update to column i, write d[i] and generate column i's reflector into A, vn, e, tau.
+// Rows [i, n) are held in registers (kPer per thread) when they fit: one round of loads,
+// then only workgroup reductions; longer columns take the same steps in several passes.
 __global__ void __launch_bounds__(kVecThreads)
     k_trd_vec(double* __restrict__ A, int64_t lda, int n, int i, const double* __restrict__ vp,
-              double* __restrict__ wp, const double* __restrict__ y, double* __restrict__ vn,
+              double* __restrict__ wp, const double* __restrict__ y,
+              const double* __restrict__ part, int nslots, double* __restrict__ vn,
               double* __restrict__ d, double* __restrict__ e, double* __restrict__ tau)
 {
     __shared__ double red[kVecThreads / 64];
+    __shared__ double bc[2];
     double* __restrict__ col = A + (int64_t)i * lda;
+    const int s = i, m = n - i;  // rows of the previous column's y: [i, n)
+    if (n - i <= kVecThreads * kPer) {
+        double cv[kPer], yv[kPer], vv[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int r = i + threadIdx.x + k * kVecThreads;
+            const bool in = r < n;
+            cv[k] = in ? col[r] : 0.0;
+            yv[k] = (in && i > 0) ? y_at(y, part, nslots, m, s, r) : 0.0;
+            vv[k] = (in && i > 0) ? vp[r] : 0.0;
+        }
+        if (i > 0) {
+            // x = tau_p y, w = x - (tau_p/2)(x.v) v over rows [i, n)
+            const double tp = tau[i - 1];
+            double dot = 0.0;
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) dot += (tp * yv[k]) * vv[k];
+            dot = block_sum(dot, red);
+            const double alpha = -0.5 * tp * dot;
+            if (threadIdx.x == 0) {
+                bc[0] = vv[0];                       // v[i]
+                bc[1] = tp * yv[0] + alpha * vv[0];  // w[i]
+            }
+            __syncthreads();
+            const double vi = bc[0], wi = bc[1];
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int r = i + threadIdx.x + k * kVecThreads;
+                const double w = tp * yv[k] + alpha * vv[k];
+                if (r < n) wp[r] = w;
+                cv[k] -= vv[k] * wi + w * vi;
+            }
+        }
+        if (threadIdx.x == 0) {
+            d[i] = cv[0];
+            col[i] = cv[0];
+        }
+        if (i >= n - 1) return;
+        // dlarfg on alpha = A[i+1, i] (thread 1's first row), x = rows [i+2, n)
+        __syncthreads();
+        if (threadIdx.x == 1) bc[0] = cv[0];
+        __syncthreads();
+        const double alpha = bc[0];
+        double mx = 0.0;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int r = i + threadIdx.x + k * kVecThreads;
+            if (r >= i + 2 && r < n) mx = fmax(mx, fabs(cv[k]));
+        }
+        mx = block_max(mx, red);
+        double ss = 0.0;
+        if (mx > 0.0) {
+            const double inv = 1.0 / mx;
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int r = i + threadIdx.x + k * kVecThreads;
+                const double t = (r >= i + 2 && r < n) ? cv[k] * inv : 0.0;
+                ss += t * t;
+            }
+        }
+        ss = block_sum(ss, red);
+        const double xnorm = mx * sqrt(ss);
+        if (xnorm == 0.0) {
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int r = i + threadIdx.x + k * kVecThreads;
+                if (r >= i + 1 && r < n) {
+                    col[r] = cv[k];
+                    vn[r] = r == i + 1 ? 1.0 : 0.0;
+                }
+            }
+            if (threadIdx.x == 0) {
+                e[i] = alpha;
+                tau[i] = 0.0;
+            }
+            return;
+        }
+        const double beta = -copysign(hypot(alpha, xnorm), alpha);
+        const double sc = 1.0 / (alpha - beta);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int r = i + threadIdx.x + k * kVecThreads;
+            if (r >= i + 2 && r < n) {
+                const double v = cv[k] * sc;
+                col[r] = v;
+                vn[r] = v;
+            }
+        }
+        if (threadIdx.x == 0) {
+            vn[i + 1] = 1.0;
+            col[i + 1] = beta;
+            e[i] = beta;
+            tau[i] = (beta - alpha) / beta;
+        }
+        return;
+    }
+    // long columns: the same steps, one pass over memory each
     if (i > 0) {
-        // previous column p = i - 1, rows [i, n): x = tau_p y, w = x - (tau_p/2)(x.v) v
         const double tp = tau[i - 1];
         double dot = 0.0;
-        for (int r = i + threadIdx.x; r < n; r += blockDim.x) dot += (tp * y[r]) * vp[r];
+        for (int r = i + threadIdx.x; r < n; r += blockDim.x)
+            dot += (tp * y_at(y, part, nslots, m, s, r)) * vp[r];
         dot = block_sum(dot, red);
         const double alpha = -0.5 * tp * dot;
-        for (int r = i + threadIdx.x; r < n; r += blockDim.x) wp[r] = tp * y[r] + alpha * vp[r];
+        for (int r = i + threadIdx.x; r < n; r += blockDim.x)
+            wp[r] = tp * y_at(y, part, nslots, m, s, r) + alpha * vp[r];
         __syncthreads();
         const double vi = vp[i], wi = wp[i];
         for (int r = i + threadIdx.x; r < n; r += blockDim.x)
@@ -169,7 +289,6 @@ __global__ void __launch_bounds__(kVecThreads)
     }
     if (threadIdx.x == 0) d[i] = col[i];
     if (i >= n - 1) return;
-    // dlarfg on alpha = A[i+1, i], x = A[i+2 .., i]
     const double alpha = col[i + 1];
     double mx = 0.0;
     for (int r = i + 2 + threadIdx.x; r < n; r += blockDim.x) mx = fmax(mx, fabs(col[r]));
@@ -193,7 +312,6 @@ __global__ void __launch_bounds__(kVecThreads)
         return;
     }
     const double beta = -copysign(hypot(alpha, xnorm), alpha);
-    const double t = (beta - alpha) / beta;
     const double sc = 1.0 / (alpha - beta);
     __syncthreads();  // every thread has read col[i + 1] and x before they change
     for (int r = i + 2 + threadIdx.x; r < n; r += blockDim.x) {
@@ -205,7 +323,7 @@ __global__ void __launch_bounds__(kVecThreads)
         vn[i + 1] = 1.0;
         col[i + 1] = beta;
         e[i] = beta;
-        tau[i] = t;
+        tau[i] = (beta - alpha) / beta;
     }
 }
 
@@ -283,8 +401,68 @@ __global__ void __launch_bounds__(256)
     if (r == 0) z[i + 1] = 1.0;
     if (r >= m) return;
     double acc = 0.0;
+#pragma unroll 8
     for (int t = 0; t < nslots; ++t) acc += part[(int64_t)t * m + r];
     z[s + r] = acc;
+}
+
+// Blocked accumulation of the explicit factor (compact WY, LAPACK dlarft / dlarfb):
+// reflectors j0 .. j0+nb-1 form H(j0) ... H(j0+nb-1) = I - V T V^T, applied to the trailing
+// block Q[j0+1.., j0+1..] with three library GEMMs (W = V^T Q, W = T W, Q -= V W).
+constexpr int kNB = 64;           // reflectors per block
+constexpr int kBlockedMin = 256;  // smaller orders take the unblocked path
+
+// V (m x nb, ld m) for the block starting at reflector j0: rows [j0+1, n), column c holds
+// v_{j0+c} (zero above its unit entry).
+__global__ void __launch_bounds__(256)
+    k_org_vblock(const double* __restrict__ A, int64_t lda, int n, int j0, int nb,
+                 double* __restrict__ V)
+{
+    const int m = n - j0 - 1;
+    const int64_t tot = (int64_t)m * nb;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(q % m), c = (int)(q / m);
+        const int row = j0 + 1 + r, j = j0 + c;
+        V[q] = row < j + 1 ? 0.0 : row == j + 1 ? 1.0 : A[row + (int64_t)j * lda];
+    }
+}
+
+// T (nb x nb, ld nb, upper) from S = V^T V and tau, column by column as dlarft
+// (forward, columnwise): T[j, j] = tau_j, T[0:j, j] = T[0:j, 0:j] (-tau_j S[0:j, j]).
+__global__ void __launch_bounds__(kNB)
+    k_org_tfactor(const double* __restrict__ S, const double* __restrict__ tau, int j0, int nb,
+                  double* __restrict__ T)
+{
+    __shared__ double t[kNB][kNB + 1];   // t[col][row]
+    __shared__ double sv[kNB][kNB + 1];  // S staged: sv[col][row]
+    __shared__ double tv[kNB];
+    const int i = threadIdx.x;
+    // stage S and tau once (the column loop below then touches LDS only)
+    for (int j = 0; j < nb; ++j)
+        if (i < nb) sv[j][i] = S[i + j * nb];
+    if (i < nb) tv[i] = tau[j0 + i];
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+        const double tj = tv[j];
+        if (i < j) {
+            double acc = 0.0;
+            for (int k = i; k < j; ++k) acc += t[k][i] * (-tj * sv[j][k]);
+            t[j][i] = acc;
+        } else if (i < nb) {
+            t[j][i] = i == j ? tj : 0.0;
+        }
+        __syncthreads();
+    }
+    for (int j = 0; j < nb; ++j)
+        if (i < nb) T[i + j * nb] = t[j][i];
+}
+
+int blas_status(rocblas_status st, const char* what)
+{
+    if (st == rocblas_status_success) return SEP_OK;
+    set_error("sep_orgtr: %s -> rocBLAS status %d", what, (int)st);
+    return SEP_ERR_HIP;
 }
 
 int check_args(const char* who, int64_t n, int64_t lda, const void* A)
@@ -301,6 +479,70 @@ int check_args(const char* who, int64_t n, int64_t lda, const void* A)
 }
 
 }  // namespace
+
+namespace {
+
+// Q <- H(0) ... H(n-2) (Q = I on entry), blocks of kNB reflectors from the last one.
+int orgtr_blocked(sep_handle* h, int n, const double* A, int64_t lda, const double* tau,
+                  double* Q, int64_t ldq)
+{
+    hipStream_t st = h->stream;
+    if (!h->blas) {
+        rocblas_handle b = nullptr;
+        if (int rc = blas_status(rocblas_create_handle(&b), "create handle")) return rc;
+        h->blas = b;
+    }
+    rocblas_handle bh = static_cast<rocblas_handle>(h->blas);
+    if (int rc = blas_status(rocblas_set_stream(bh, st), "set stream")) return rc;
+    const size_t nb = kNB;
+    // V (n x nb), W and W2 (nb x n), S and T (nb x nb)
+    if (int rc = h->trd_blk.reserve(sizeof(double) * (3 * nb * (size_t)n + 2 * nb * nb)))
+        return rc;
+    double* V = h->trd_blk.as<double>();
+    double* W = V + nb * (size_t)n;
+    double* W2 = W + nb * (size_t)n;
+    double* S = W2 + nb * (size_t)n;
+    double* T = S + nb * nb;
+    const double one = 1.0, zero = 0.0, mone = -1.0;
+    for (int jend = n - 1; jend > 0;) {
+        const int j0 = std::max(0, jend - kNB), b = jend - j0, m = n - j0 - 1;
+        hipLaunchKernelGGL(k_org_vblock, dim3((unsigned)std::min<int64_t>(2048, ((int64_t)m * b + 255) / 256)),
+                           dim3(256), 0, st, A, lda, n, j0, b, V);
+        SEP_CHECK_LAUNCH();
+        if (int rc = blas_status(rocblas_dgemm(bh, rocblas_operation_transpose,
+                                               rocblas_operation_none, b, b, m, &one, V, m, V, m,
+                                               &zero, S, b),
+                                 "V^T V"))
+            return rc;
+        hipLaunchKernelGGL(k_org_tfactor, dim3(1), dim3(kNB), 0, st, S, tau, j0, b, T);
+        SEP_CHECK_LAUNCH();
+        double* Qr = Q + (j0 + 1) + (int64_t)(j0 + 1) * ldq;
+        if (int rc = blas_status(rocblas_dgemm(bh, rocblas_operation_transpose,
+                                               rocblas_operation_none, b, m, m, &one, V, m, Qr,
+                                               (rocblas_int)ldq, &zero, W, b),
+                                 "V^T Q"))
+            return rc;
+        if (int rc = blas_status(rocblas_dgemm(bh, rocblas_operation_none, rocblas_operation_none,
+                                               b, m, b, &one, T, b, W, b, &zero, W2, b),
+                                 "T W"))
+            return rc;
+        if (int rc = blas_status(rocblas_dgemm(bh, rocblas_operation_none, rocblas_operation_none,
+                                               m, m, b, &mone, V, m, W2, b, &one, Qr,
+                                               (rocblas_int)ldq),
+                                 "Q - V W"))
+            return rc;
+        jend = j0;
+    }
+    return SEP_OK;
+}
+
+}  // namespace
+
+void release_blas(sep_handle* h)
+{
+    if (h->blas) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(h->blas));
+    h->blas = nullptr;
+}
 
 int sytrd_f64(sep_handle* h, int64_t n64, double* A, int64_t lda, double* d, double* e,
               double* tau)
@@ -325,19 +567,29 @@ int sytrd_f64(sep_handle* h, int64_t n64, double* A, int64_t lda, double* d, dou
     double* w = h->trd_vec.as<double>() + 2 * (size_t)n;
     double* y = h->trd_vec.as<double>() + 3 * (size_t)n;
     double* part = h->trd_part.as<double>();
+    // partials of the previous column's y when few enough to be summed in k_trd_vec
+    const double* fpart = nullptr;
+    int fslots = 0;
     for (int i = 0; i < n; ++i) {
         // column i: previous column's w and update, d[i], reflector i (into v[i & 1])
         hipLaunchKernelGGL(k_trd_vec, dim3(1), dim3(kVecThreads), 0, st, A, lda, n, i,
-                           v[(i + 1) & 1], w, y, v[i & 1], d, e, tau);
+                           v[(i + 1) & 1], w, y, fpart, fslots, v[i & 1], d, e, tau);
         SEP_CHECK_LAUNCH();
         if (i >= n - 1) break;
         const int s = i + 1, m = n - s, nt = (m + kTile - 1) / kTile;
         hipLaunchKernelGGL(k_trd_tiles, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(kThreads), 0,
                            st, A, lda, s, m, v[(i + 1) & 1], w, i > 0 ? 1 : 0, v[i & 1], part);
         SEP_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_trd_reduce, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st,
-                           part, nt, s, m, y);
-        SEP_CHECK_LAUNCH();
+        if (nt <= kFuseSlots) {
+            fpart = part;
+            fslots = nt;
+        } else {
+            fpart = nullptr;
+            fslots = 0;
+            hipLaunchKernelGGL(k_trd_reduce, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st,
+                               part, nt, s, m, y);
+            SEP_CHECK_LAUNCH();
+        }
     }
     return SEP_OK;
 }
@@ -358,6 +610,7 @@ int orgtr_f64(sep_handle* h, int64_t n64, const double* A, int64_t lda, const do
                        dim3(256), 0, st, Q, ldq, n);
     SEP_CHECK_LAUNCH();
     if (n == 1) return SEP_OK;
+    if (n >= kBlockedMin) return orgtr_blocked(h, n, A, lda, tau, Q, ldq);
     const int ntmax = (n - 1 + kTile - 1) / kTile;
     if (int rc = h->trd_vec.reserve(sizeof(double) * 4 * (size_t)n)) return rc;
     if (int rc = h->trd_part.reserve(sizeof(double) * (size_t)ntmax * (size_t)(n - 1))) return rc;
