@@ -14,10 +14,10 @@ Also reported in the same line:
   src/bulk_givens.jl) timed on this host's cores on a row sample of the same workload;
 * extra: the other apply shapes (n = 2000 Float64/Float32, n = 16384 Float32), the plain
   solve of the README example (n = 2000) beside the oracle's CPU solve, the single large
-  solve (n = 16384, max_layers = 64), the
-  differentiated-solve workload (n = 4096: forward solve, backward, and their ratio), the
-  dense-input reduction (n = 4096: sytrd against its HBM floor, orgtr), and the batched
-  workload (4096 problems: solves/s against the per-problem GPU loop).
+  solve (n = 16384, max_layers = 64), the differentiated-solve workload (n = 4096: forward
+  solve, backward, and their ratio; and the backward batched over 1024 problems of order
+  64..256), the dense-input reduction (n = 4096: sytrd against its HBM floor, orgtr), and
+  the batched workload (4096 problems: solves/s against the per-problem GPU loop).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
                       [--calibrate-copy-gib G]
@@ -182,6 +182,46 @@ def extras(sep, h, torch, np, cpu=True, large=True):
     ex["ad_n4096_backward_tflops"] = round(4.0 * n ** 3 / bwd / 1e12, 2)
     log(f"[bench] AD n=4096: forward {fwd:.3f} s, backward {bwd * 1e3:.2f} ms "
         f"({ex['ad_n4096_backward_tflops']} TFLOP/s on 4 n^3), ratio {bwd / fwd:.5f}")
+    del lam, Q, Gd
+
+    # config 5, batched: the same backward over 1024 problems of order 64..256 (forward by
+    # the batched solver, eigenvalues as it leaves them; the cotangents are N(0, 1))
+    Bb = 1024
+    rb = np.random.default_rng(11)
+    nsb = rb.integers(64, 257, size=Bb).astype(np.int64)
+    nsb += nsb % 2  # even orders (the solver's precondition)
+    db, eb, offb, offqb = sep.inputs.batched_problems(nsb, seed=13)
+    totq = int(offqb[-1] + nsb[-1] ** 2)
+    fwd_b = float("inf")
+    for _ in range(2):  # the first call also captures the solver's graph
+        dvb = torch.from_numpy(db.copy()).cuda()
+        evb = torch.from_numpy(eb.copy()).cuda()
+        Qb = torch.zeros(totq, dtype=torch.float64, device="cuda")
+        for p, nn in enumerate(nsb):
+            Qb[offqb[p]:offqb[p] + nn * nn].view(nn, nn).fill_diagonal_(1.0)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        sep.qr_batched_(nsb, offb, offqb, dvb, evb, Qb, handle=h)
+        torch.cuda.synchronize()
+        fwd_b = min(fwd_b, time.perf_counter() - t)
+    lbar = torch.from_numpy(rb.standard_normal(dvb.shape[0])).cuda()
+    qbar = torch.from_numpy(rb.standard_normal(totq)).cuda()
+    sep.eig_backward_batched(nsb, offb, offqb, dvb, Qb, lbar, qbar, handle=h)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        sep.eig_backward_batched(nsb, offb, offqb, dvb, Qb, lbar, qbar, handle=h)
+    torch.cuda.synchronize()
+    bwd_b = (time.perf_counter() - t) / reps
+    fl = 4.0 * float((nsb.astype(np.float64) ** 3).sum())
+    ex["ad_batched_1024_forward_s"] = round(fwd_b, 4)
+    ex["ad_batched_1024_backward_s"] = round(bwd_b, 5)
+    ex["ad_batched_1024_bwd_over_fwd"] = round(bwd_b / fwd_b, 4)
+    ex["ad_batched_1024_backward_tflops"] = round(fl / bwd_b / 1e12, 2)
+    log(f"[bench] AD batched 1024 x n in [64, 256]: forward {fwd_b * 1e3:.1f} ms (default "
+        f"configuration), backward {bwd_b * 1e3:.2f} ms ({ex['ad_batched_1024_backward_tflops']} "
+        f"TFLOP/s on 4 sum n^3), ratio {bwd_b / fwd_b:.3f}")
+    del dvb, evb, Qb, lbar, qbar
 
     # dense symmetric input (no reference counterpart): reduction to tridiagonal and the
     # explicit orthogonal factor at n = 4096, A = (X + X^T) / 2 with X ~ N(0, 1)
