@@ -467,7 +467,8 @@ int blas_status(rocblas_status st, const char* what)
 
 int check_args(const char* who, int64_t n, int64_t lda, const void* A)
 {
-    if (n < 0 || n > 65536 || lda < std::max<int64_t>(n, 1)) {
+    // leading dimensions reach the blocked path's library GEMMs as 32-bit integers
+    if (n < 0 || n > 65536 || lda < std::max<int64_t>(n, 1) || lda > INT32_MAX) {
         set_error("%s: invalid n=%lld ld=%lld", who, (long long)n, (long long)lda);
         return SEP_ERR_ARG;
     }

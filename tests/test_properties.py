@@ -167,3 +167,32 @@ def test_gpu_solve_random_bitwise(sep, handle, half_n, max_layers, seed):
     assert conv == oconv and it == oit and nrot == onrot, (n, max_layers, seed)
     assert np.array_equal(H.dv, od) and np.array_equal(H.ev, oe), (n, max_layers, seed)
     assert np.linalg.norm(Q - oQ) <= 1e-13 * np.sqrt(n), (n, max_layers, seed)
+
+
+@pytest.mark.gpu
+@settings(max_examples=30, deadline=None, derandomize=True,
+          suppress_health_check=[HealthCheck.too_slow, HealthCheck.function_scoped_fixture])
+@given(st.integers(1, 400), st.integers(0, 7), st.integers(0, 2 ** 31))
+def test_gpu_sytrd_orgtr_random(sep, handle, n, pad, seed):
+    """Dense -> tridiagonal over random orders (both the unblocked and the blocked factor,
+    which switch at n = 256) and padded leading dimensions: d, e against LAPACK dsytrd to
+    1e-12 |A|, and Q^T A Q = T, Q^T Q = I with the GPU's own factor."""
+    import torch
+    from tridiag_ref import lapack_sytrd, sym_matrix, tridiag
+    A = sym_matrix(n, seed)
+    _, d_ref, e_ref, _ = lapack_sytrd(A)
+    ld = n + pad
+    big = torch.zeros((n, ld), dtype=torch.float64, device="cuda")
+    big[:, :n] = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()
+    Ad = big.t()[:n, :]
+    d, e, tau = sep.sytrd_(Ad, handle=handle)
+    Q = sep.to_host_colmajor(sep.orgtr(Ad, tau, handle=handle))
+    d, e = d.cpu().numpy(), e.cpu().numpy()
+    nA = max(np.linalg.norm(A), 1e-300)
+    assert np.abs(d - d_ref).max() <= 1e-12 * nA, (n, pad, seed)
+    if n > 1:
+        assert np.abs(e - e_ref).max() <= 1e-12 * nA, (n, pad, seed)
+    assert np.linalg.norm(Q.T @ Q - np.eye(n)) <= 4e-14 * n, (n, pad, seed)
+    assert np.linalg.norm(Q.T @ A @ Q - tridiag(d, e)) <= 4e-14 * n * nA, (n, pad, seed)
+    if pad:
+        assert torch.all(big[:, n:] == 0.0), (n, pad, seed)
