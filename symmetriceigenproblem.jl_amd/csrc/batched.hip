@@ -41,6 +41,11 @@ constexpr int kStageSteps = 16;  // wave-order time steps staged per LDS fill
 constexpr int kQAhead = 8;       // Q columns prefetched ahead
 constexpr int kMaxBatchedN = 8192;
 constexpr int kGraphRounds = 4;  // rounds per captured graph (even: buffer parity repeats)
+// The batched chase keeps the per-step scalar range test of the rotation: with several chase
+// waves (and the apply) sharing each SIMD, vector-unit issue is the scarce resource, and the
+// deferred test moves work from the scalar to the vector unit (measured with eigenvectors:
+// 80-81.5 ms per 4096-problem solve with the scalar test, 85 ms deferred).
+constexpr bool kBatchedDeferRange = false;
 
 // state word (beyond the chase's own) marking a finished problem
 constexpr int ST_DONE = 10;
@@ -90,7 +95,7 @@ __global__ void __launch_bounds__(kLanes)
     chase::Problem<double> P{dv + pr.off_d, ev + pr.off_d, lt + pr.off_lt, grid + 2 * pr.off_g,
                              pr.n, ML, (int64_t)(pr.n - 1), 0};
     const double tol = 2.220446049250313e-16;  // eps(Float64)
-    chase::chase_block(P, tol, 100 * (int64_t)pr.n, st, lane);
+    chase::chase_block<double, kBatchedDeferRange>(P, tol, 100 * (int64_t)pr.n, st, lane);
     __threadfence_block();
     if (lane == 0) {
         // maxiter exceeded: the reference returns without applying this block

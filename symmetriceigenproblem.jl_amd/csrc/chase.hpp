@@ -13,7 +13,12 @@
 //    changes) for the next chunk while the current one runs, the recurrence reads them
 //    with v_readlane and deposits its finished outputs (ev[i-1], dv[i], c, s) back in
 //    lane j with a lane select, and the chunk is stored with one coalesced store per
-//    array.  The recurrence itself therefore never waits on memory.
+//    array.  The recurrence itself therefore never waits on memory;
+//  * in Float64 the rotation's range test (extreme magnitudes take the complete DLARTG)
+//    can be deferred to the end of a chunk: each step folds its operands' exponent words
+//    into a running maximum, and a chunk that left the common range is run again from its
+//    entry state with the per-step test (single solves defer; the batched chase, where
+//    vector issue is shared by several waves, keeps the per-step scalar test).
 // Arithmetic follows the reference operation by operation without contraction, and
 // uses IEEE division and square root, so the grid, dv, ev and iteration count equal
 // the C restatement's bit for bit.
@@ -22,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace sep {
 namespace chase {
@@ -222,19 +228,10 @@ __device__ __forceinline__ double xor_sign(double x, uint32_t m)
 //    reach RN(1/r) as the compiler's division sequence does, so the quotients are the
 //    same correctly rounded values, and the reciprocal no longer waits for r's final
 //    correction before it starts.
-template <>
-__device__ __forceinline__ Rot<double> givens<double>(double f, double g)
+// The common-path arithmetic of givens<double> for sign-rule mask sm (0, or 0x80000000 when
+// |f| > |g| and f < 0).
+__device__ __forceinline__ Rot<double> rot_common(double f, double g, uint32_t sm)
 {
-    const uint32_t fh = __builtin_amdgcn_readfirstlane((int)hi_word(f));
-    const uint32_t gh = __builtin_amdgcn_readfirstlane((int)hi_word(g));
-    // hi(|x|) in (hi(2^-380), hi(2^485)) for both, as one unsigned compare of the larger
-    // offset: (h - lo - 1) wraps around for h <= lo
-    constexpr uint32_t kLo = 0x28300000u, kHi = 0x5E400000u;  // 2^-380, 2^485
-    const uint32_t fo = (fh & 0x7fffffffu) - (kLo + 1), go = (gh & 0x7fffffffu) - (kLo + 1);
-    const bool rare = (fo > go ? fo : go) >= kHi - (kLo + 1);
-    // sign rule: |f| > |g| (one compare, read as a wave ballot) and f < 0 (fh's sign bit)
-    const bool gt = __ballot(tabs(f) > tabs(g)) != 0;
-    const uint32_t sm = (gt ? fh : 0u) & 0x80000000u;
     // square root of t = f^2 + g^2 (as sqrt_common), keeping the half-reciprocal estimate
     const double t = f * f + g * g;
     const double y = __builtin_amdgcn_rsq(t);
@@ -260,9 +257,77 @@ __device__ __forceinline__ Rot<double> givens<double>(double f, double g)
     o.c = __builtin_fma(r1, yy, q1);
     o.s = __builtin_fma(r2, yy, q2);
     o.r = rr;
+    return o;
+}
+
+// Bounds of the common path: hi(|x|) in (hi(2^-380), hi(2^485)) for both operands.
+constexpr uint32_t kRangeLo = 0x28300000u, kRangeHi = 0x5E400000u;
+
+// Float64 specialisation of the same rotation with the bookkeeping moved off the vector
+// unit.  The values are wave-uniform, so:
+//  * the range test runs on the scalar unit from the high words: |x| < 2^485 exactly when
+//    hi(|x|) < hi(2^485), and hi(|x|) > hi(2^-380) implies |x| > 2^-380 (the few values
+//    just above 2^-380 that fail it take the complete path, which is exact for them too);
+//  * |f| > |g| is one vector compare read as a wave ballot, f < 0 is the sign bit;
+//  * the sign rule (c, s, r) -> -(c, s, r) is applied to r before dividing, since
+//    f / (-r) == -(f / r) exactly;
+//  * the reciprocal of r is refined from twice the square root's half-reciprocal estimate
+//    h ~ 1/(2 sqrt t) instead of a separate v_rcp_f64: two Newton steps from that seed
+//    reach RN(1/r) as the compiler's division sequence does, so the quotients are the
+//    same correctly rounded values, and the reciprocal no longer waits for r's final
+//    correction before it starts.
+template <>
+__device__ __forceinline__ Rot<double> givens<double>(double f, double g)
+{
+    const uint32_t fh = __builtin_amdgcn_readfirstlane((int)hi_word(f));
+    const uint32_t gh = __builtin_amdgcn_readfirstlane((int)hi_word(g));
+    // one unsigned compare of the larger offset: (h - lo - 1) wraps around for h <= lo
+    const uint32_t fo = (fh & 0x7fffffffu) - (kRangeLo + 1),
+                   go = (gh & 0x7fffffffu) - (kRangeLo + 1);
+    const bool rare = (fo > go ? fo : go) >= kRangeHi - (kRangeLo + 1);
+    // sign rule: |f| > |g| (one compare, read as a wave ballot) and f < 0 (fh's sign bit)
+    const bool gt = __ballot(tabs(f) > tabs(g)) != 0;
+    const uint32_t sm = (gt ? fh : 0u) & 0x80000000u;
+    Rot<double> o = rot_common(f, g, sm);
     asm volatile("" ::"v"(o.c), "v"(o.s), "v"(o.r));
     if (__builtin_expect(rare, 0)) o = givens_slow(f, g);
     return o;
+}
+
+// The same rotation with its range test deferred: no scalar round trip and no branch per
+// step (a single chase wave waits on the vector-to-scalar transfer of each step's operands
+// and on the branch: with the deferred test the step takes ~141 ns instead of ~182 ns).  acc collects 2 (hi(|x|) - hi(2^-380) - 1) mod 2^32 of both operands (the offset
+// of the test above, doubled so that the shift drops the sign bit); a chunk of steps stayed
+// in the common range iff acc < kRareAcc at its end, and is otherwise run again with
+// givens().  The sign rule is evaluated per lane (the values are the same in every lane).
+constexpr uint32_t kRareAcc = 2u * (kRangeHi - (kRangeLo + 1));
+__device__ __forceinline__ uint32_t range_word(double x)
+{
+    // (hi << 1) + (-2 (lo + 1)) as one shift-and-add (the compiler would otherwise take
+    // the shifted word from the 64-bit value with a funnel shift and a mask)
+    uint32_t r;
+    asm("v_lshl_add_u32 %0, %1, 1, %2"
+        : "=v"(r)
+        : "v"(hi_word(x)), "s"(0u - 2u * (kRangeLo + 1)));
+    return r;
+}
+__device__ __forceinline__ Rot<double> givens_deferred(double f, double g, uint32_t& acc)
+{
+    acc = max(acc, max(range_word(f), range_word(g)));
+    const uint32_t sm = tabs(f) > tabs(g) ? (hi_word(f) & 0x80000000u) : 0u;
+    return rot_common(f, g, sm);
+}
+
+// The chase step's rotation: with the range test deferred (Float64 chunks) or complete.
+__device__ __forceinline__ Rot<double> step_rotation(std::true_type, double f, double g,
+                                                     uint32_t& acc)
+{
+    return givens_deferred(f, g, acc);
+}
+template <typename T>
+__device__ __forceinline__ Rot<T> step_rotation(std::false_type, T f, T g, uint32_t&)
+{
+    return givens(f, g);
 }
 
 // is_offdiagonal_small, src/bulge_chasing.jl:7-8 (1-based i; dv, ev 0-based arrays)
@@ -395,7 +460,7 @@ __device__ int max_thickness(const Problem<T>& P, int from, int to, int lane)
 // single_shift!(H, from, to, mu, givens, layer_thickness), src/bulge_chasing.jl:11-144,
 // executed by the whole wave (all lanes compute the uniform recurrence; lane j owns the
 // memory traffic of step i0 + j in each 64-step chunk).
-template <typename T>
+template <typename T, bool kDeferRange = true>
 __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int lane)
 {
     T* __restrict__ dv = P.dv;
@@ -468,27 +533,44 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
             const int cnt = __builtin_amdgcn_readfirstlane(min(kWave, i_last - i0 + 1));
             // outputs of step i0 + j are kept in lane j and stored once per chunk
             T oR = T(0), oD = T(0), oC = T(0), oS = T(0);
-            uint64_t jm = 1;  // lane mask of step j
-            for (int j = 0; j < cnt; ++j, jm <<= 1) {
-                const T B22 = rdlane(D, j);
-                const Rot<T> G = givens(p1, bulge);
-                const T c = G.c, s = G.s;
-                const T B11 = d_cur, B21 = e_cur;
-                const T B11p = __builtin_fma(c, B11, s * B21);
-                const T B21p = __builtin_fma(-s, B11, c * B21);
-                const T B12p = __builtin_fma(c, B21, s * B22);
-                const T B22p = __builtin_fma(-s, B21, c * B22);
-                const T B11pp = __builtin_fma(B11p, c, B12p * s);
-                const T B12pp = __builtin_fma(B11p, -s, B12p * c);
-                const T B22pp = __builtin_fma(B21p, -s, B22p * c);
-                capture_lane(oR, oD, oC, oS, G.r, B11pp, c, s, jm);
-                p1 = B12pp;
-                d_cur = B22pp;
-                // B23 = ev[i+1]; the last step of the sweep has none: its prefetch slot
-                // holds 0, and the bulge / e_cur it then produces are never read
-                const T B23 = rdlane(E, j);
-                bulge = s * B23;
-                e_cur = c * B23;
+            // the chunk's entry state, for the rerun of a chunk that left the common range
+            const T p1_0 = p1, d_0 = d_cur, e_0 = e_cur, b_0 = bulge;
+            uint32_t acc = 0;
+            auto run = [&](auto deferred) {
+                uint64_t jm = 1;  // lane mask of step j
+                for (int j = 0; j < cnt; ++j, jm <<= 1) {
+                    const T B22 = rdlane(D, j);
+                    const Rot<T> G = step_rotation(deferred, p1, bulge, acc);
+                    const T c = G.c, s = G.s;
+                    const T B11 = d_cur, B21 = e_cur;
+                    const T B11p = __builtin_fma(c, B11, s * B21);
+                    const T B21p = __builtin_fma(-s, B11, c * B21);
+                    const T B12p = __builtin_fma(c, B21, s * B22);
+                    const T B22p = __builtin_fma(-s, B21, c * B22);
+                    const T B11pp = __builtin_fma(B11p, c, B12p * s);
+                    const T B12pp = __builtin_fma(B11p, -s, B12p * c);
+                    const T B22pp = __builtin_fma(B21p, -s, B22p * c);
+                    capture_lane(oR, oD, oC, oS, G.r, B11pp, c, s, jm);
+                    p1 = B12pp;
+                    d_cur = B22pp;
+                    // B23 = ev[i+1]; the last step of the sweep has none: its prefetch slot
+                    // holds 0, and the bulge / e_cur it then produces are never read
+                    const T B23 = rdlane(E, j);
+                    bulge = s * B23;
+                    e_cur = c * B23;
+                }
+            };
+            if constexpr (std::is_same<T, double>::value && kDeferRange) {
+                run(std::true_type{});
+                if (__builtin_expect(__builtin_amdgcn_readfirstlane((int)(acc >= kRareAcc)), 0)) {
+                    p1 = p1_0;
+                    d_cur = d_0;
+                    e_cur = e_0;
+                    bulge = b_0;
+                    run(std::false_type{});
+                }
+            } else {
+                run(std::false_type{});
             }
             // coalesced stores of the chunk; the next chunk was prefetched from entries
             // these stores do not touch (dv/ev beyond i0 + 62, lt beyond i0 + 62)
@@ -523,7 +605,7 @@ enum StateWord {
 
 // One outer iteration of qr_algorithm! without the apply (src/bulge_chasing.jl:166-276):
 // layer_thickness is reset here; the grid must hold identity rotations on entry.
-template <typename T>
+template <typename T, bool kDeferRange = true>
 __device__ void chase_block(const Problem<T>& P, T tol, int64_t maxiter, int64_t* st, int lane)
 {
     const int start = 1;
@@ -556,7 +638,7 @@ __device__ void chase_block(const Problem<T>& P, T tol, int64_t maxiter, int64_t
             } else {
                 const T lam = wilkinson(P.dv[to - 2], P.ev[to - 2], P.dv[to - 1]);
                 did = 1;
-                single_shift(P, from, to, lam, lane);
+                single_shift<T, kDeferRange>(P, from, to, lam, lane);
                 // the next scan and sweep read what this sweep's lanes stored
                 __threadfence_block();
                 nrot += to - from;
