@@ -41,11 +41,9 @@ constexpr int kStageSteps = 16;  // wave-order time steps staged per LDS fill
 constexpr int kQAhead = 8;       // Q columns prefetched ahead
 constexpr int kMaxBatchedN = 8192;
 constexpr int kGraphRounds = 4;  // rounds per captured graph (even: buffer parity repeats)
-// The batched chase keeps the per-step scalar range test of the rotation: with several chase
-// waves (and the apply) sharing each SIMD, vector-unit issue is the scarce resource, and the
-// deferred test moves work from the scalar to the vector unit (measured with eigenvectors:
-// 80-81.5 ms per 4096-problem solve with the scalar test, 85 ms deferred).
-constexpr bool kBatchedDeferRange = false;
+// The batched chase shares each SIMD with other chase waves and the apply: it takes the
+// chase's shared-SIMD form (per-step scalar range test, chunk inputs read from LDS).
+constexpr bool kBatchedSolo = false;
 
 // state word (beyond the chase's own) marking a finished problem
 constexpr int ST_DONE = 10;
@@ -95,7 +93,7 @@ __global__ void __launch_bounds__(kLanes)
     chase::Problem<double> P{dv + pr.off_d, ev + pr.off_d, lt + pr.off_lt, grid + 2 * pr.off_g,
                              pr.n, ML, (int64_t)(pr.n - 1), 0};
     const double tol = 2.220446049250313e-16;  // eps(Float64)
-    chase::chase_block<double, kBatchedDeferRange>(P, tol, 100 * (int64_t)pr.n, st, lane);
+    chase::chase_block<double, kBatchedSolo>(P, tol, 100 * (int64_t)pr.n, st, lane);
     __threadfence_block();
     if (lane == 0) {
         // maxiter exceeded: the reference returns without applying this block
@@ -435,7 +433,9 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
     hipEvent_t* applied = h->batch_ev + 2;
     const int nwork = (int)L.work.size();
     const int64_t max_rounds = 100 * (int64_t)L.nmax + 3;
-    const unsigned reserve = cfg.CW >= 4 ? 0u : (unsigned)(kLdsPerCU / (4 * cfg.CW));
+    // (the chase's own static LDS counts towards the share)
+    const unsigned reserve =
+        cfg.CW >= 4 ? 0u : (unsigned)(kLdsPerCU / (4 * cfg.CW)) - chase::kChaseLdsBytes;
     const int acap = cfg.AW > 0 ? cfg.AW * h->num_cus * 4 : 0;  // apply waves (AW per SIMD)
     // Round r: the chase (chase stream) waits for the apply of round r - 2, which used the
     // same buffers; the apply (apply stream) waits for the chase; the finished count is
