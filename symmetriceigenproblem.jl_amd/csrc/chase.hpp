@@ -246,11 +246,12 @@ __device__ __forceinline__ Rot<double> rot_common(double f, double g, uint32_t s
     d = __builtin_fma(-gq, gq, t);
     gq = __builtin_fma(d, h, gq);
     const double rr = xor_sign(gq, sm);  // +-RN(sqrt(t))
-    // two quotients over rr with the shared, Newton-refined reciprocal
+    // two quotients over rr with the shared, Newton-refined reciprocal: y0 is already
+    // accurate to ~eps_rsq^2 (h went through a Newton step above), as the compiler's
+    // division sequence has it after its first step from v_rcp_f64, so one step brings it
+    // to rounding level, as the compiler's second step does
     double yy = y0;
-    double ee = __builtin_fma(-rr, yy, 1.0);
-    yy = __builtin_fma(yy, ee, yy);
-    ee = __builtin_fma(-rr, yy, 1.0);
+    const double ee = __builtin_fma(-rr, yy, 1.0);
     yy = __builtin_fma(yy, ee, yy);
     const double q1 = f * yy, q2 = g * yy;
     const double r1 = __builtin_fma(-rr, q1, f), r2 = __builtin_fma(-rr, q2, g);
