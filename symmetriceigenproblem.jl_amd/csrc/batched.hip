@@ -63,13 +63,24 @@ struct BWork {
     int row0;
 };
 
+// A problem's dv, ev and layer counts in LDS take this many bytes per order (beside the
+// LDS share that caps the chase waves per SIMD); problems up to kLdsChaseMaxN chase there.
+constexpr unsigned kLdsBytesPerN = 2 * sizeof(double) + sizeof(int);
+constexpr int kLdsChaseMaxN = 3072;
+
 // One block of sweeps for every unfinished problem (one wavefront each).  lt / grid /
-// depth are this round's buffers.
+// depth are this round's buffers.  With in_lds the wave first copies its problem's dv and
+// ev into dynamic LDS (nmax entries each, then nmax layer counts) and runs the block there:
+// the deflation scans, thickness maxima and the sweeps' loads and stores then never wait
+// on a global memory that the concurrent apply keeps busy.  The block's results are
+// copied back (dv, ev and the layer counts the apply reads); the grid goes straight to
+// global memory.
 __global__ void __launch_bounds__(kLanes)
     k_chase_all(const BProb* __restrict__ probs, const int* __restrict__ order, double* dv,
                 double* ev, int* lt, double* grid, int4* info, int64_t* st_all, int ML, int nmax,
-                int* done_count)
+                int* done_count, int in_lds)
 {
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
     // blocks are dispatched in index order as wave slots free up; block b takes the b-th
     // largest problem, so the longest blocks start first (longest-first list scheduling)
     const int p = order[blockIdx.x];
@@ -93,7 +104,26 @@ __global__ void __launch_bounds__(kLanes)
     chase::Problem<double> P{dv + pr.off_d, ev + pr.off_d, lt + pr.off_lt, grid + 2 * pr.off_g,
                              pr.n, ML, (int64_t)(pr.n - 1), 0};
     const double tol = 2.220446049250313e-16;  // eps(Float64)
-    chase::chase_block<double, kBatchedSolo>(P, tol, 100 * (int64_t)pr.n, st, lane);
+    if (in_lds) {
+        double* ld = reinterpret_cast<double*>(dyn_lds);
+        double* le = ld + nmax;
+        int* ll = reinterpret_cast<int*>(le + nmax);
+        for (int q = lane; q < pr.n; q += kLanes) {
+            ld[q] = P.dv[q];
+            if (q + 1 < pr.n) le[q] = P.ev[q];
+        }
+        __syncthreads();
+        chase::Problem<double> PL{ld, le, ll, P.grid, pr.n, ML, (int64_t)(pr.n - 1), 0};
+        chase::chase_block<double, kBatchedSolo>(PL, tol, 100 * (int64_t)pr.n, st, lane);
+        __syncthreads();
+        for (int q = lane; q < pr.n; q += kLanes) {
+            P.dv[q] = ld[q];
+            if (q + 1 < pr.n) P.ev[q] = le[q];
+            P.lt[q] = ll[q];
+        }
+    } else {
+        chase::chase_block<double, kBatchedSolo>(P, tol, 100 * (int64_t)pr.n, st, lane);
+    }
     __threadfence_block();
     if (lane == 0) {
         // maxiter exceeded: the reference returns without applying this block
@@ -433,9 +463,16 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
     hipEvent_t* applied = h->batch_ev + 2;
     const int nwork = (int)L.work.size();
     const int64_t max_rounds = 100 * (int64_t)L.nmax + 3;
-    // (the chase's own static LDS counts towards the share)
-    const unsigned reserve =
+    // (the chase's own static LDS counts towards the share).  Beside the apply, problems of
+    // order up to kLdsChaseMaxN are chased in LDS, which needs kLdsBytesPerN bytes per order
+    // of the largest problem (4096 problems of order 32..512: 68-70 ms against 70-74 ms
+    // from global memory); for eigenvalues only, with nothing else loading memory, the
+    // chase reads global memory (36 ms against 39-45 ms through LDS)
+    const int in_lds = (Q != nullptr && L.nmax <= kLdsChaseMaxN) ? 1 : 0;
+    const unsigned occ =
         cfg.CW >= 4 ? 0u : (unsigned)(kLdsPerCU / (4 * cfg.CW)) - chase::kChaseLdsBytes;
+    const unsigned need = in_lds ? (kLdsBytesPerN * (unsigned)L.nmax + 15u) & ~15u : 0u;
+    const unsigned reserve = std::max(occ, need);
     const int acap = cfg.AW > 0 ? cfg.AW * h->num_cus * 4 : 0;  // apply waves (AW per SIMD)
     // Round r: the chase (chase stream) waits for the apply of round r - 2, which used the
     // same buffers; the apply (apply stream) waits for the chase; the finished count is
@@ -447,7 +484,7 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
         if (!first && (e = hipStreamWaitEvent(cs, applied[b], 0)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_chase_all, dim3((unsigned)B), dim3(kLanes), reserve, cs, d_probs,
                            d_order, dv, ev, d_lt[b], d_grid[b], d_depth[b], d_st, cfg.ML, L.nmax,
-                           d_count);
+                           d_count, in_lds);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = hipMemcpyAsync(pin, d_count, sizeof(int), hipMemcpyDeviceToHost, cs)) !=
                 hipSuccess ||
