@@ -96,25 +96,52 @@ __global__ void __launch_bounds__(256) k_colsigns(const T* __restrict__ cs, int6
                                                   uint64_t* __restrict__ mask,
                                                   T* __restrict__ sigm1)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    if (*flag) {
+    // 64 columns per block, four threads per column, each over 16 of the (at most 64)
+    // layers: the chunk's sign bits as if the column entered it positive, and the chunk's
+    // parity; the chunks are then joined in layer order (a chunk entered negative has all
+    // its bits inverted)
+    __shared__ uint64_t cm[4][64];
+    __shared__ int cp[4][64];
+    const int c = threadIdx.x % 64, q = threadIdx.x / 64;
+    const int j = blockIdx.x * 64 + c;
+    const bool general = *flag != 0;
+    bool neg = false;
+    uint64_t m = 0;
+    if (j < n && !general) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int lr = q * 16 + u, L = L0 + lr;
+            if (lr < Kp && L < k) {
+                if (neg) m |= 1ull << lr;
+                // rotations (j-1, L) then (j, L) touch column j in layer L
+                neg ^= neg_c<T>(cs, ld, n, j - 1, L, k) ^ neg_c<T>(cs, ld, n, j, L, k);
+            }
+        }
+    }
+    cm[q][c] = m;
+    cp[q][c] = neg;
+    __syncthreads();
+    if (q != 0 || j >= n) return;
+    if (general) {
         mask[j] = 0;
         sigm1[j] = 0.0;
         return;
     }
-    bool neg = false;
-    uint64_t m = 0;
-    for (int lr = 0; lr < Kp; ++lr) {
-        const int L = L0 + lr;
-        if (L >= k) break;
-        if (neg) m |= 1ull << lr;
-        // rotations (j-1, L) then (j, L) touch column j in layer L
-        neg ^= neg_c<T>(cs, ld, n, j - 1, L, k) ^ neg_c<T>(cs, ld, n, j, L, k);
+    uint64_t mm = 0;
+    bool pre = false;
+    for (int t = 0; t < 4; ++t) {
+        const uint64_t valid = ((t * 16 + 16 <= Kp) ? 0xFFFFull
+                                                    : ((1ull << (Kp > t * 16 ? Kp - t * 16 : 0)) - 1))
+                               << (16 * t);
+        const uint64_t lim = (L0 + t * 16 < k)
+                                 ? (((k - L0) >= 64) ? ~0ull : ((1ull << (k - L0)) - 1))
+                                 : 0ull;
+        mm |= pre ? ((cm[t][c] ^ valid) & lim & valid) : cm[t][c];
+        pre ^= cp[t][c] != 0;
     }
-    if (in_fix && neg) m = ~m;
-    mask[j] = m;
-    sigm1[j] = neg ? -2.0 : 0.0;
+    if (in_fix && pre) mm = ~mm;
+    mask[j] = mm;
+    sigm1[j] = pre ? -2.0 : 0.0;
 }
 
 // Coefficient table of one pass: tab[g][u][2 KG] for g < G, u = t + S + PADU (time t,
@@ -662,7 +689,7 @@ int run_dpp(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T
     const int64_t rb = (m + kLanes - 1) / kLanes;
     int tp_prev = tp0;
     for (int64_t L0 = 0; L0 < k; L0 += Kp) {
-        hipLaunchKernelGGL(k_colsigns<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cs,
+        hipLaunchKernelGGL(k_colsigns<T>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st, cs,
                            ld_cs, (int)n, (int)k, (int)L0, Kp, 1, flag, mask, sigm1);
         SEP_CHECK_LAUNCH();
         const int64_t total = (int64_t)G * Up * KG + (int64_t)NI * 16;
@@ -974,7 +1001,7 @@ int run_r16(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T
     const int64_t wgs = (m + Gm::RPG - 1) / Gm::RPG;
     int tp_prev = tp0;
     for (int64_t L0 = 0; L0 < k; L0 += Kp) {
-        hipLaunchKernelGGL(k_colsigns<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cs,
+        hipLaunchKernelGGL(k_colsigns<T>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st, cs,
                            ld_cs, (int)n, (int)k, (int)L0, Kp, 0, flag, mask, sigm1);
         SEP_CHECK_LAUNCH();
         const int tblocks = (int)std::min<int64_t>(((int64_t)tab_elems + 255) / 256, 8192);
