@@ -206,3 +206,12 @@ def test_odd_orders_extension(sep):
             assert np.array_equal(Qg, Qo), (p, n)
     finally:
         h.close()
+
+
+@pytest.mark.parametrize("ns", [[3072, 1024, 64, 8, 2], [3100, 600, 32]])
+def test_rounds_large_orders(sep, handle, ns):
+    """Problems up to order 3072 are chased in LDS beside the apply; a batch whose largest
+    order exceeds that chases from global memory: both bit for bit against the oracle."""
+    ns, d, e, off_d, off_q = _batch(sep, ns, seed=int(sum(ns)))
+    conv, iters, gd, ge, gQ = _solve(sep, handle, ns, d, e, off_d, off_q, max_layers=32)
+    _check_bitwise(ns, d, e, off_d, off_q, 32, conv, iters, gd, ge, gQ)
