@@ -162,6 +162,28 @@ __device__ __forceinline__ double y_at(const double* __restrict__ y,
     return acc;
 }
 
+// Workgroup maximum of mx and sum of sq together (both reduced in a fixed order).
+__device__ void block_max_sum(double& mx, double& sq, double* red, double* red2)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        mx = fmax(mx, __shfl_xor(mx, off, 64));
+        sq += __shfl_xor(sq, off, 64);
+    }
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    __syncthreads();
+    if (lane == 0) {
+        red[w] = mx;
+        red2[w] = sq;
+    }
+    __syncthreads();
+    mx = 0.0;
+    sq = 0.0;
+    for (int k = 0; k < (int)(blockDim.x / 64); ++k) {
+        mx = fmax(mx, red[k]);
+        sq += red2[k];
+    }
+}
+
 // Column i: finish the previous column's w (from y = A22 v of column i - 1), apply its
 // update to column i, write d[i] and generate column i's reflector into A, vn, e, tau.
 // Rows [i, n) are held in registers (kPer per thread) when they fit: one round of loads,
@@ -172,7 +194,7 @@ __global__ void __launch_bounds__(kVecThreads)
               const double* __restrict__ part, int nslots, double* __restrict__ vn,
               double* __restrict__ d, double* __restrict__ e, double* __restrict__ tau)
 {
-    __shared__ double red[kVecThreads / 64];
+    __shared__ double red[kVecThreads / 64], red2[kVecThreads / 64];
     __shared__ double bc[2];
     double* __restrict__ col = A + (int64_t)i * lda;
     const int s = i, m = n - i;  // rows of the previous column's y: [i, n)
@@ -218,25 +240,33 @@ __global__ void __launch_bounds__(kVecThreads)
         if (threadIdx.x == 1) bc[0] = cv[0];
         __syncthreads();
         const double alpha = bc[0];
-        double mx = 0.0;
+        // |x|: the largest magnitude and the plain sum of squares in one reduction; the sum
+        // is rescaled by the largest magnitude only when squares could leave the range
+        double mx = 0.0, sq = 0.0;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int r = i + threadIdx.x + k * kVecThreads;
-            if (r >= i + 2 && r < n) mx = fmax(mx, fabs(cv[k]));
+            if (r >= i + 2 && r < n) {
+                mx = fmax(mx, fabs(cv[k]));
+                sq += cv[k] * cv[k];
+            }
         }
-        mx = block_max(mx, red);
-        double ss = 0.0;
-        if (mx > 0.0) {
+        block_max_sum(mx, sq, red, red2);
+        double xnorm;
+        if (mx == 0.0 || (mx > 0x1p-400 && mx < 0x1p400)) {
+            xnorm = sqrt(sq);
+        } else {
             const double inv = 1.0 / mx;
+            double ss = 0.0;
 #pragma unroll
             for (int k = 0; k < kPer; ++k) {
                 const int r = i + threadIdx.x + k * kVecThreads;
                 const double t = (r >= i + 2 && r < n) ? cv[k] * inv : 0.0;
                 ss += t * t;
             }
+            ss = block_sum(ss, red);
+            xnorm = mx * sqrt(ss);
         }
-        ss = block_sum(ss, red);
-        const double xnorm = mx * sqrt(ss);
         if (xnorm == 0.0) {
 #pragma unroll
             for (int k = 0; k < kPer; ++k) {

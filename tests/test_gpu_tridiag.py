@@ -118,3 +118,19 @@ def test_errors(sep, handle):
         sep.SEP_ERR_ARG
     assert L.sep_sytrd_f64_dev(None, 4, At.data_ptr(), 4, d.data_ptr(), d.data_ptr(),
                                d.data_ptr()) == sep.SEP_ERR_ARG
+
+
+@pytest.mark.parametrize("scale_exp", [600, -600])
+def test_sytrd_extreme_magnitudes(sep, handle, scale_exp):
+    """A scaled by 2^+-600, where the squares in the reflector norm leave the range and the
+    scaled norm is taken: d, e against LAPACK dsytrd of the same matrix, 1e-12 relative."""
+    n = 150
+    A0 = sym_matrix(n, seed=31)
+    A = A0 * 2.0 ** scale_exp
+    _, d_ref, e_ref, _ = lapack_sytrd(A)
+    G = A.copy(order="F")
+    d, e, tau = sep.sytrd_(G, handle=handle)
+    nA = np.linalg.norm(A0) * 2.0 ** scale_exp  # (the norm of A itself would underflow)
+    assert np.all(np.isfinite(d)) and np.all(np.isfinite(e))
+    assert np.abs(d - d_ref).max() <= 1e-12 * nA
+    assert np.abs(e - e_ref).max() <= 1e-12 * nA
