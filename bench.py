@@ -84,6 +84,17 @@ def cpu_baseline(n: int, k: int, rows: int, threads: int, reps: int):
     return flops / best / 1e9, best
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def time_apply(sep, h, torch, n, k, dtype, steps=5):
     gen = torch.Generator(device="cuda").manual_seed(5)
     A = torch.randn((n, n), generator=gen, dtype=dtype, device="cuda").t()
@@ -403,6 +414,7 @@ def main() -> int:
         },
     }
 
+    cpu_extra = {}
     if not args.no_cpu and f64:
         threads = min(16, os.cpu_count() or 1)
         log(f"[bench] cpu baseline: oracle wave-order apply, {args.cpu_rows} rows x n={n}, "
@@ -414,6 +426,17 @@ def main() -> int:
                       f"min of {args.cpu_reps} runs of {best:.3f} s",
         }
         log(f"[bench] cpu {gfs:.1f} GFLOP/s")
+        # the single-thread figure (the analogue of bench beside bench_parallel) on a
+        # 1024-row sample, and the host it was measured on
+        gfs1, best1 = cpu_baseline(n, k, 1024, 1, 1)
+        cpu_extra = {
+            "cpu_apply_1thread_gflops": round(gfs1, 2),
+            "cpu_apply_1thread_sample": f"1024 rows x n={n}, k={k}, one run of {best1:.3f} s",
+            "cpu_host_nproc": os.cpu_count(),
+            "cpu_host_model": cpu_model(),
+        }
+        log(f"[bench] cpu 1 thread: {gfs1:.1f} GFLOP/s ({cpu_extra['cpu_host_model']}, "
+            f"{cpu_extra['cpu_host_nproc']} logical CPUs)")
 
     if args.calibrate_copy_gib > 0:
         nel = int(args.calibrate_copy_gib * (1 << 30)) // 8
@@ -427,6 +450,8 @@ def main() -> int:
 
     if not args.no_extra:
         out["extra"] = extras(sep, h, torch, np, cpu=not args.no_cpu, large=not args.no_large)
+    if cpu_extra:
+        out.setdefault("extra", {}).update(cpu_extra)
 
     print(json.dumps(out), flush=True)
     h.close()
