@@ -327,7 +327,7 @@ def main() -> int:
     ap.add_argument("--no-large", action="store_true",
                     help="skip the n = 16384 solve in the extras (about 40 s)")
     ap.add_argument("--cpu-rows", type=int, default=16384)
-    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--calibrate-copy-gib", type=float, default=0.0,
                     help="after the timed steps, run one plain device copy of this many GiB "
                          "(the reference kernel for FETCH_SIZE / WRITE_SIZE calibration)")
@@ -428,10 +428,11 @@ def main() -> int:
         log(f"[bench] cpu {gfs:.1f} GFLOP/s")
         # the single-thread figure (the analogue of bench beside bench_parallel) on a
         # 1024-row sample, and the host it was measured on
-        gfs1, best1 = cpu_baseline(n, k, 1024, 1, 1)
+        gfs1, best1 = cpu_baseline(n, k, 1024, 1, args.cpu_reps)
         cpu_extra = {
             "cpu_apply_1thread_gflops": round(gfs1, 2),
-            "cpu_apply_1thread_sample": f"1024 rows x n={n}, k={k}, one run of {best1:.3f} s",
+            "cpu_apply_1thread_sample": f"1024 rows x n={n}, k={k}, min of {args.cpu_reps} runs of "
+                                        f"{best1:.3f} s",
             "cpu_host_nproc": os.cpu_count(),
             "cpu_host_model": cpu_model(),
         }
