@@ -38,6 +38,9 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix) peak
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak
 HBM_PEAK_GBS = 8000.0
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "apply_dpp_n16384_k64_f64_pmc.md")
+# Published bench_parallel(T, 2000, 2000, 64) figures of the reference README (README.md:43-47;
+# 8 threads on the author's CPU), GFLOP/s -- context for the n = 2000 extras, not the headline
+PUBLISHED_N2000_8T = {"apply_n2000_f64": 267.13, "apply_n2000_f32": 530.83}
 
 
 def pmc_traffic_bytes():
@@ -121,6 +124,9 @@ def extras(sep, h, torch, np, cpu=True, large=True):
         peak = FP64_PEAK_TFLOPS if dt == torch.float64 else FP32_PEAK_TFLOPS
         ex[name + "_gflops"] = round(gfs, 1)
         ex[name + "_frac_of_vector_peak"] = round(gfs / 1e3 / peak, 4)
+        if name in PUBLISHED_N2000_8T:
+            # the reference README's bench_parallel figure (8 threads, its own CPU)
+            ex[name + "_over_published_8thread_cpu"] = round(gfs / PUBLISHED_N2000_8T[name], 1)
         log(f"[bench] {name}: {gfs:.1f} GFLOP/s ({ms:.3f} ms/step)")
 
     # config 1: README example solve, n = 2000, max_layers = 128
@@ -135,6 +141,8 @@ def extras(sep, h, torch, np, cpu=True, large=True):
     ex["solve_n2000_iter"] = it
     ex["solve_n2000_rotations"] = nrot
     ex["solve_n2000_apply_gflops"] = round(6.0 * 2000 * nrot / ex["solve_n2000_s"] / 1e9, 1)
+    # the reference README's qr_algorithm! at n = 2000 (its own CPU, unseeded input): 0.258 s
+    ex["solve_n2000_over_published_cpu"] = round(0.258 / ex["solve_n2000_s"], 2)
     log(f"[bench] solve n=2000: {ex['solve_n2000_s']} s, iter={it}, rotations={nrot}")
     if cpu:
         import oracle
