@@ -134,3 +134,23 @@ def test_sytrd_extreme_magnitudes(sep, handle, scale_exp):
     assert np.all(np.isfinite(d)) and np.all(np.isfinite(e))
     assert np.abs(d - d_ref).max() <= 1e-12 * nA
     assert np.abs(e - e_ref).max() <= 1e-12 * nA
+
+
+def test_dense_long_columns_on_device(sep, handle):
+    """n = 8320: the first columns exceed the 8192 rows the column step keeps in registers
+    and take its multi-pass form.  Checked on the device: Q^T Q = I and Q^T A Q = T."""
+    import torch
+    n = 8320
+    gen = torch.Generator(device="cuda").manual_seed(17)
+    X = torch.randn((n, n), generator=gen, dtype=torch.float64, device="cuda")
+    A = (X + X.T) / 2
+    del X
+    Ad = A.clone().t().contiguous().t()  # column-major copy (A is symmetric)
+    d, e, tau = sep.sytrd_(Ad, handle=handle)
+    Q = sep.orgtr(Ad, tau, handle=handle)
+    del Ad
+    nA = torch.linalg.norm(A).item()
+    I = torch.eye(n, dtype=torch.float64, device="cuda")
+    assert torch.linalg.norm(Q.T @ Q - I).item() <= 4e-14 * n
+    T = torch.diag(d) + torch.diag(e, 1) + torch.diag(e, -1)
+    assert torch.linalg.norm(Q.T @ (A @ Q) - T).item() <= 4e-14 * n * nA
