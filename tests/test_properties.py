@@ -196,3 +196,58 @@ def test_gpu_sytrd_orgtr_random(sep, handle, n, pad, seed):
     assert np.linalg.norm(Q.T @ A @ Q - tridiag(d, e)) <= 4e-14 * n * nA, (n, pad, seed)
     if pad:
         assert torch.all(big[:, n:] == 0.0), (n, pad, seed)
+
+
+@pytest.mark.gpu
+@settings(max_examples=25, deadline=None, derandomize=True,
+          suppress_health_check=[HealthCheck.too_slow, HealthCheck.function_scoped_fixture])
+@given(st.lists(st.integers(1, 80), min_size=1, max_size=24), st.sampled_from([7, 16, 33]),
+       st.integers(0, 2 ** 31))
+def test_gpu_batched_random_bitwise(sep, handle, halves, max_layers, seed):
+    """Random batches (even orders 2..160, 1..24 problems) and block lengths: every
+    problem's dv, ev, Q, iteration count and convergence flag equal the oracle's
+    qr_algorithm with the two-FMA rotation, bit for bit."""
+    import torch
+    ns = (2 * np.asarray(halves)).astype(np.int64)
+    d, e, off_d, off_q = sep.inputs.batched_problems(ns, seed=seed % 100000)
+    tot_q = int(off_q[-1] + ns[-1] ** 2)
+    dvd = torch.from_numpy(d.copy()).cuda()
+    evd = torch.from_numpy(e.copy()).cuda()
+    Q = torch.zeros(tot_q, dtype=torch.float64, device="cuda")
+    for p, n in enumerate(ns):
+        Q[off_q[p]:off_q[p] + n * n].view(n, n).fill_diagonal_(1.0)
+    conv, iters = sep.qr_batched_(ns, off_d, off_q, dvd, evd, Q, max_layers, handle=handle)
+    gd, ge, gQ = dvd.cpu().numpy(), evd.cpu().numpy(), Q.cpu().numpy()
+    ml = max_layers
+    for p, n in enumerate(ns):
+        n, o = int(n), off_d[p]
+        dv, ev = d[o:o + n].copy(), e[o:o + n - 1].copy()
+        Qo = np.asfortranarray(np.eye(n))
+        oconv, oit, _, _ = oracle.qr_algorithm(dv, ev, Qo, ml, apply_mode=3)
+        assert bool(conv[p]) == oconv and iters[p] == oit, (p, n, ml)
+        assert np.array_equal(gd[o:o + n], dv) and np.array_equal(ge[o:o + n - 1], ev), (p, n)
+        assert np.array_equal(gQ[off_q[p]:off_q[p] + n * n].reshape(n, n).T, Qo), (p, n)
+
+
+@pytest.mark.gpu
+@settings(max_examples=25, deadline=None, derandomize=True,
+          suppress_health_check=[HealthCheck.too_slow, HealthCheck.function_scoped_fixture])
+@given(st.integers(2, 300), st.integers(0, 2 ** 31))
+def test_gpu_backward_random(sep, handle, n, seed):
+    """Random orders (d_i = i + small noise, e in [0.1, 1): well separated spectra) and
+    cotangents: the GPU backward against the oracle's dense formula, 1e-11 relative."""
+    import torch
+    r = np.random.default_rng(seed)
+    d = np.arange(1.0, n + 1) + 0.05 * r.standard_normal(n)
+    e = 0.1 + 0.9 * r.random(n - 1)
+    w = r.standard_normal(n)
+    G = np.asfortranarray(r.standard_normal((n, n)))
+    lam, Q = oracle.eig_sorted_signfixed(d, e)
+    rd, re = oracle.eig_backward_dense(lam, Q, w, G)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    db, eb = sep.eig_backward(dev(lam), sep.to_device_colmajor(Q), dev(w),
+                              sep.to_device_colmajor(G), handle=handle)
+    db, eb = db.cpu().numpy(), eb.cpu().numpy()
+    scale = max(np.abs(rd).max(), np.abs(re).max())
+    assert np.abs(db - rd).max() <= 1e-11 * scale, (n, seed)
+    assert np.abs(eb - re).max() <= 1e-11 * scale, (n, seed)
