@@ -160,6 +160,20 @@ int sep_eig_backward_batched_f64_dev(sep_handle *h, int64_t B, const int64_t *n,
                                      const double *lam, const double *Q, const double *lam_bar,
                                      const double *Q_bar, double *d_bar, double *e_bar);
 
+/* ---- Sorted output (the convenience beside eigen_decomp!) ------------------------
+ * qr_algorithm! leaves the eigenvalues in H.dv in deflation order (src/bulge_chasing.jl:
+ * 146-288) and eigen_decomp! returns them so (:290-299).  This puts them in ascending
+ * order: lam_out[r] = lam[perm[r]] and column r of Q_out = column perm[r] of Q, ties
+ * broken by index (NaNs last, negative NaNs first), so A Q_out = Q_out diag(lam_out)
+ * holds whenever A Q = Q diag(lam).  Out of place (Q_out must not overlap Q); Q = NULL
+ * sorts the eigenvalues only; perm (n int64, 0-based) may be NULL.  Device pointers, on
+ * the handle's stream. */
+int sep_sort_eigen_f64_dev(sep_handle *h, int64_t n, const double *lam, const double *Q,
+                           int64_t ldq, double *lam_out, double *Q_out, int64_t ldo,
+                           int64_t *perm);
+int sep_sort_eigen_f32_dev(sep_handle *h, int64_t n, const float *lam, const float *Q,
+                           int64_t ldq, float *lam_out, float *Q_out, int64_t ldo, int64_t *perm);
+
 /* ---- Dense symmetric -> tridiagonal (no counterpart in the reference) ----------------
  * The reference's qr_algorithm! starts from a SymTridiagonal (README.md:3); these two
  * entries put the reduction in front of it for a dense symmetric A, with LAPACK's

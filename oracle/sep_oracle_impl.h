@@ -204,17 +204,40 @@ static inline void FN(rmul_one)(int64_t m, T *A, int64_t lda, int64_t i, T c, T 
     }
 }
 
+/* The reference's identity skip.  bulk_wave_order_2x2_rmul! (src/wave.jl:62-162) hands
+ * the grid to its kernels in fused blocks and skips a block only when every s in it is
+ * zero (src/wave.jl:83,98,116,137,152).  A rotation with s == 0 and c != 1 is therefore
+ * applied (it scales or negates its columns) unless its whole block is s-zero.  The
+ * block of rotation (i, L), 1-based: wave i' = start_col + j + 1 pairs layers j (odd) and
+ * j + 1 and takes rows start_col + 1, start_col + 2 of layer j and start_col,
+ * start_col + 1 of layer j + 1 (src/wave.jl:78-81), with start_col even, and the edge
+ * kernels drop the rows outside 1..n-1 (src/wave.jl:92-95,131-134).  So for L odd the
+ * block has start_col = 2 floor((i-1)/2), for L even start_col = 2 floor(i/2).  Every
+ * (i, L) lies in exactly one block when the wave preconditions hold (n-1 odd, k even,
+ * k <= n-2); the same blocks are used for any other shape. */
+static int FN(block_skipped)(const T *cs, int64_t n, int64_t i, int64_t L, int64_t k)
+{
+    const int64_t j = (L % 2 == 1) ? L : L - 1;
+    const int64_t sc = (L % 2 == 1) ? 2 * ((i - 1) / 2) : 2 * (i / 2);
+    const int64_t rows[4] = {sc + 1, sc + 2, sc, sc + 1};
+    const int64_t lays[4] = {j, j, j + 1, j + 1};
+    for (int q = 0; q < 4; q++) {
+        if (rows[q] < 1 || rows[q] > n - 1 || lays[q] > k) continue;
+        if (cs[2 * ((rows[q] - 1) + (n - 1) * (lays[q] - 1)) + 1] != 0) return 0;
+    }
+    return 1;
+}
+
 /* Layer-major sequential application of an (n-1) x k grid: the product
  * A * prod_{L=1..k} prod_{i=1..n-1} G(i,L)^T that bulk_wave_order_2x2_rmul!
- * (src/wave.jl:62-162) computes.  Identity rotations (s == 0) are skipped, as the
- * wave kernels skip all-identity fused blocks (src/wave.jl:83,116,152). */
+ * (src/wave.jl:62-162) computes, with its fused-block identity skip (block_skipped). */
 void FN(orc_bulk_rmul_seq)(int64_t m, int64_t n, T *A, int64_t lda, const T *cs, int64_t k)
 {
     for (int64_t L = 1; L <= k; L++)
         for (int64_t i = 1; i <= n - 1; i++) {
             T c = cs[2 * ((i - 1) + (n - 1) * (L - 1))];
             T s = cs[2 * ((i - 1) + (n - 1) * (L - 1)) + 1];
-            if (s == 0) continue;
+            if (FN(block_skipped)(cs, n, i, L, k)) continue;
             FN(rmul_one)(m, A, lda, i, c, s);
         }
 }
@@ -240,7 +263,7 @@ void FN(orc_bulk_rmul_seq_fma)(int64_t m, int64_t n, T *A, int64_t lda, const T 
         for (int64_t i = 1; i <= n - 1; i++) {
             T c = cs[2 * ((i - 1) + (n - 1) * (L - 1))];
             T s = cs[2 * ((i - 1) + (n - 1) * (L - 1)) + 1];
-            if (s == 0) continue;
+            if (FN(block_skipped)(cs, n, i, L, k)) continue;
             FN(rmul_one_fma)(m, A, lda, i, c, s);
         }
 }

@@ -25,7 +25,9 @@ missing or if no gfx950 device is present.
 from __future__ import annotations
 
 import ctypes as C
+import logging
 import os
+import warnings
 from dataclasses import dataclass
 from typing import Optional, Tuple
 
@@ -56,6 +58,7 @@ EXPORTS = (
     "sep_set_odd_order",
     "sep_eig_backward_f64_dev", "sep_eig_backward_batched_f64_dev",
     "sep_sytrd_f64_dev", "sep_orgtr_f64_dev",
+    "sep_sort_eigen_f64_dev", "sep_sort_eigen_f32_dev",
 )
 
 _lib: Optional[C.CDLL] = None
@@ -112,6 +115,9 @@ def _declare(L: C.CDLL) -> None:
     L.sep_eig_backward_batched_f64_dev.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P]
     L.sep_sytrd_f64_dev.argtypes = [_P, _I64, _P, _I64, _P, _P, _P]
     L.sep_orgtr_f64_dev.argtypes = [_P, _I64, _P, _I64, _P, _P, _I64]
+    for suf in ("f64", "f32"):
+        getattr(L, f"sep_sort_eigen_{suf}_dev").argtypes = [_P, _I64, _P, _P, _I64, _P, _P,
+                                                             _I64, _P]
     for name in EXPORTS:
         if name not in ("sep_last_error", "sep_version"):
             getattr(L, name).restype = C.c_int
@@ -216,6 +222,25 @@ def _np_colmajor(A: np.ndarray) -> Tuple[int, int, int]:
     return A.shape[0], A.shape[1], max(A.shape[0], 1)
 
 
+def _require_device(name: str, t, dtype=None, device=None, min_len: Optional[int] = None,
+                    vector: bool = False):
+    """Raise ValueError unless t is a torch tensor on a GPU (the *_dev entry points take
+    its data pointer), with the given dtype and device, stride 1 if a vector, and at least
+    min_len elements."""
+    if not _is_torch(t):
+        raise ValueError(f"{name} must be a torch device tensor")
+    if t.device.type != "cuda":
+        raise ValueError(f"{name} is on {t.device}; the device entry points need a GPU tensor")
+    if dtype is not None and t.dtype != dtype:
+        raise ValueError(f"{name} has dtype {t.dtype}, expected {dtype}")
+    if device is not None and t.device != device:
+        raise ValueError(f"{name} is on {t.device}, expected {device}")
+    if vector and (t.dim() != 1 or (t.numel() > 1 and t.stride(0) != 1)):
+        raise ValueError(f"{name} must be a 1-d tensor with stride 1")
+    if min_len is not None and t.numel() < min_len:
+        raise ValueError(f"{name} has {t.numel()} elements, needs at least {min_len}")
+
+
 def _torch_colmajor(A) -> Tuple[int, int, int]:
     if A.dim() != 2 or A.stride(0) != 1 or (A.shape[1] > 1 and A.stride(1) < A.shape[0]):
         raise ValueError("device matrix must be column-major: strides (1, ld) with ld >= rows")
@@ -278,7 +303,6 @@ def qr_algorithm_(H: SymTridiagonal, Q=None, max_layers: int = 128, tol=None,
     Mutates H.dv (eigenvalues, deflation order), H.ev and Q (Q <- Q * prod G^T) in place.
     Q may be None (eigenvalues only).  numpy inputs run through the host-pointer entry
     point, torch device tensors through the *_dev entry point."""
-    h = handle or default_handle()
     n = H.n
     suf = _dtype_suffix(H.dv.dtype)
     npdt = np.float64 if suf == "f64" else np.float32
@@ -286,32 +310,40 @@ def qr_algorithm_(H: SymTridiagonal, Q=None, max_layers: int = 128, tol=None,
         tol = float(np.finfo(npdt).eps)
     if maxiter is None:
         maxiter = 100 * n
-    conv = C.c_int32()
-    it = C.c_int64()
-    nrot = C.c_int64()
     dev = _is_torch(H.dv)
     if dev:
+        _require_device("H.dv", H.dv, vector=True)
+        _require_device("H.ev", H.ev, dtype=H.dv.dtype, device=H.dv.device, vector=True)
         if Q is not None:
+            _require_device("Q", Q, dtype=H.dv.dtype, device=H.dv.device)
             qm, qn, ldq = _torch_colmajor(Q)
-            if (qm, qn) != (n, n) or str(Q.dtype) != str(H.dv.dtype):
+            if (qm, qn) != (n, n):
                 raise ValueError("Q must be n x n with H's dtype")
-            qp = Q.data_ptr()
-        else:
-            qp, ldq = None, max(n, 1)
-        fn = getattr(lib(), f"sep_qr_algorithm_{suf}_dev")
-        rc = fn(h.ptr, n, H.dv.data_ptr(), H.ev.data_ptr() if n > 1 else None, qp, ldq,
-                int(max_layers), tol, int(maxiter), C.byref(conv), C.byref(it), C.byref(nrot))
     else:
         for a in (H.dv, H.ev):
             if not (isinstance(a, np.ndarray) and a.flags.c_contiguous and a.dtype == npdt):
                 raise ValueError("dv/ev must be contiguous numpy arrays of one float dtype")
         if Q is not None:
+            if _is_torch(Q):
+                raise ValueError("Q must be a numpy array when H holds numpy arrays")
             qm, qn, ldq = _np_colmajor(Q)
             if (qm, qn) != (n, n) or Q.dtype != npdt:
                 raise ValueError("Q must be n x n with H's dtype")
-            qp = Q.ctypes.data
-        else:
-            qp, ldq = None, max(n, 1)
+    h = handle or default_handle()
+    conv = C.c_int32()
+    it = C.c_int64()
+    nrot = C.c_int64()
+    if dev:
+        qp = Q.data_ptr() if Q is not None else None
+        if Q is None:
+            ldq = max(n, 1)
+        fn = getattr(lib(), f"sep_qr_algorithm_{suf}_dev")
+        rc = fn(h.ptr, n, H.dv.data_ptr(), H.ev.data_ptr() if n > 1 else None, qp, ldq,
+                int(max_layers), tol, int(maxiter), C.byref(conv), C.byref(it), C.byref(nrot))
+    else:
+        qp = Q.ctypes.data if Q is not None else None
+        if Q is None:
+            ldq = max(n, 1)
         fn = getattr(lib(), f"sep_qr_algorithm_{suf}")
         rc = fn(h.ptr, n, H.dv.ctypes.data, H.ev.ctypes.data if n > 1 else None, qp, ldq,
                 int(max_layers), tol, int(maxiter), C.byref(conv), C.byref(it), C.byref(nrot))
@@ -324,26 +356,34 @@ def qr_algorithm_(H: SymTridiagonal, Q=None, max_layers: int = 128, tol=None,
 def bulk_wave_order_2x2_rmul_(A, givens, handle: Optional[Handle] = None):
     """bulk_wave_order_2x2_rmul!(A, givens): A <- A * prod_L prod_i G(i,L)^T in place.
 
-    givens has shape (k, n-1, 2) (a Julia Matrix{Tuple{T,T}} of size (n-1) x k).  Unlike
-    the reference there is no parity precondition on n or k."""
-    h = handle or default_handle()
+    givens has shape (k, n-1, 2) (a Julia Matrix{Tuple{T,T}} of size (n-1) x k).  A fused
+    block of rotations whose s are all zero is skipped, as in the reference (a rotation
+    with s = 0 and c != 1 elsewhere is applied).  Unlike the reference there is no parity
+    precondition on n or k."""
     if _is_torch(A):
+        _require_device("A", A)
+        _require_device("givens", givens, dtype=A.dtype, device=A.device)
         m, n, lda = _torch_colmajor(A)
         if not givens.is_contiguous():
             raise ValueError("givens must be contiguous")
         k = givens.shape[0] if givens.numel() else 0
-        if givens.numel() and (givens.shape[1] != n - 1 or givens.shape[2] != 2):
+        if givens.numel() and (givens.dim() != 3 or givens.shape[1] != n - 1
+                               or givens.shape[2] != 2):
             raise ValueError("givens must have shape (k, n-1, 2)")
         suf = _dtype_suffix(A.dtype)
+        h = handle or default_handle()
         fn = getattr(lib(), f"sep_bulk_rmul_{suf}_dev")
         _check(fn(h.ptr, m, n, A.data_ptr(), lda, givens.data_ptr(), k))
     else:
+        if _is_torch(givens):
+            raise ValueError("givens must be a numpy array when A is")
         m, n, lda = _np_colmajor(A)
+        suf = _dtype_suffix(A.dtype)
         g = np.ascontiguousarray(givens, dtype=A.dtype)
         k = g.shape[0] if g.size else 0
-        if g.size and g.shape[1:] != (n - 1, 2):
+        if g.size and (g.ndim != 3 or g.shape[1:] != (n - 1, 2)):
             raise ValueError("givens must have shape (k, n-1, 2)")
-        suf = _dtype_suffix(A.dtype)
+        h = handle or default_handle()
         fn = getattr(lib(), f"sep_bulk_rmul_{suf}")
         _check(fn(h.ptr, m, n, A.ctypes.data, lda, g.ctypes.data, k))
     return None
@@ -352,16 +392,65 @@ def bulk_wave_order_2x2_rmul_(A, givens, handle: Optional[Handle] = None):
 # The GPU entry point already spreads the rows of A over the whole device.
 bulk_wave_order_2x2_rmul_parallel_ = bulk_wave_order_2x2_rmul_
 
+_log = logging.getLogger("sep_amd")
 
-def eigen_decomp_(H: SymTridiagonal, handle: Optional[Handle] = None):
-    """eigen_decomp!(H): Q = I, qr_algorithm!(H, Q), returns (eigenvalues, Q)."""
+
+def sort_eigen(lam, Q=None, handle: Optional[Handle] = None):
+    """Eigenvalues in ascending order and Q's columns permuted to match (device tensors).
+
+    Returns (lam_sorted, Q_sorted, perm) with lam_sorted[r] = lam[perm[r]] and column r of
+    Q_sorted = column perm[r] of Q (new tensors; Q_sorted is None when Q is None).  Ties
+    keep index order; NaNs go last."""
+    import torch
+    _require_device("lam", lam, vector=True)
+    n = lam.shape[0]
+    suf = _dtype_suffix(lam.dtype)
+    ldq = max(n, 1)
+    if Q is not None:
+        _require_device("Q", Q, dtype=lam.dtype, device=lam.device)
+        qm, qn, ldq = _torch_colmajor(Q)
+        if (qm, qn) != (n, n):
+            raise ValueError("Q must be n x n")
+    h = handle or default_handle()
+    lam_out = torch.empty_like(lam)
+    perm = torch.empty(n, dtype=torch.int64, device=lam.device)
+    Q_out = colmajor_empty(n, n, lam.dtype, lam.device) if Q is not None else None
+    fn = getattr(lib(), f"sep_sort_eigen_{suf}_dev")
+    _check(fn(h.ptr, n, lam.data_ptr(), Q.data_ptr() if Q is not None else None, ldq,
+              lam_out.data_ptr(), Q_out.data_ptr() if Q is not None else None, max(n, 1),
+              perm.data_ptr()))
+    h.synchronize()
+    return lam_out, Q_out, perm
+
+
+def eigen_decomp_(H: SymTridiagonal, handle: Optional[Handle] = None, sort: bool = False,
+                  max_layers: int = 128, maxiter: Optional[int] = None):
+    """eigen_decomp!(H): Q = I, qr_algorithm!(H, Q), returns (eigenvalues, Q).
+
+    Like the reference (src/bulge_chasing.jl:290-299) the eigenvalues are H.dv in
+    deflation order and the solve's converged / iter are logged (logger "sep_amd", INFO);
+    a solve that did not converge also raises a RuntimeWarning, since its Q lacks the
+    unfinished block's rotations.  sort=True (device tensors) returns the eigenvalues in
+    ascending order with Q's columns permuted to match (sort_eigen).  max_layers and
+    maxiter pass through to qr_algorithm_ (the reference uses its defaults)."""
     n = H.n
     if _is_torch(H.dv):
+        _require_device("H.dv", H.dv, vector=True)
         import torch
         Q = torch.eye(n, dtype=H.dv.dtype, device=H.dv.device).t()  # column-major view
     else:
+        if sort:
+            raise ValueError("sort=True needs device tensors")
         Q = np.asfortranarray(np.eye(n, dtype=H.dv.dtype))
-    _, converged, it = qr_algorithm_(H, Q, handle=handle)
+    _, converged, it = qr_algorithm_(H, Q, max_layers, maxiter=maxiter, handle=handle)
+    _log.info("done converged=%s iter=%d", converged, it)
+    if not converged:
+        warnings.warn(f"qr_algorithm! did not converge within maxiter (iter={it}); the "
+                      "eigenvalues and Q are those of the last completed block",
+                      RuntimeWarning, stacklevel=2)
+    if sort:
+        lam, Qs, _ = sort_eigen(H.dv, Q, handle=handle)
+        return lam, Qs
     return H.dv, Q
 
 
@@ -369,6 +458,19 @@ def eigen_decomp_(H: SymTridiagonal, handle: Optional[Handle] = None):
 
 def _host_i64(a) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(a, dtype=np.int64))
+
+
+def _batch_layout(ns, off_d, off_q):
+    """Host int64 copies of a packed batch layout and the buffer lengths it needs
+    (dv/ev: max(off_d + n); Q: max(off_q + n^2)); raises ValueError on a malformed one."""
+    ns, off_d, off_q = _host_i64(ns), _host_i64(off_d), _host_i64(off_q)
+    if ns.ndim != 1 or off_d.shape != ns.shape or off_q.shape != ns.shape:
+        raise ValueError("ns, off_d and off_q must be 1-d arrays of one length")
+    if ns.size and (ns.min() < 0 or off_d.min() < 0 or off_q.min() < 0):
+        raise ValueError("orders and offsets must be non-negative")
+    need_d = int((off_d + ns).max()) if ns.size else 0
+    need_q = int((off_q + ns * ns).max()) if ns.size else 0
+    return ns, off_d, off_q, need_d, need_q
 
 
 def qr_batched_(ns, off_d, off_q, dv, ev, Q, max_layers: int = 0, handle: Optional[Handle] = None):
@@ -380,8 +482,15 @@ def qr_batched_(ns, off_d, off_q, dv, ev, Q, max_layers: int = 0, handle: Option
     Q may be None (eigenvalues only).
     Returns (converged[B] bool numpy, iters[B] int64 numpy)."""
     import torch
+    ns, off_d, off_q, need_d, need_q = _batch_layout(ns, off_d, off_q)
+    _require_device("dv", dv, dtype=torch.float64, min_len=need_d, vector=True)
+    _require_device("ev", ev, dtype=torch.float64, device=dv.device, min_len=need_d - 1,
+                    vector=True)
+    if Q is not None:
+        _require_device("Q", Q, dtype=torch.float64, device=dv.device, min_len=need_q)
+        if not Q.is_contiguous():
+            raise ValueError("Q must be a contiguous packed buffer")
     h = handle or default_handle()
-    ns, off_d, off_q = _host_i64(ns), _host_i64(off_d), _host_i64(off_q)
     B = ns.shape[0]
     conv = torch.zeros(B, dtype=torch.int32, device=dv.device)
     iters = torch.zeros(B, dtype=torch.int64, device=dv.device)
@@ -396,8 +505,12 @@ def qr_batched_(ns, off_d, off_q, dv, ev, Q, max_layers: int = 0, handle: Option
 
 
 def tune_batched(ns, off_d, off_q, dv, ev, handle: Optional[Handle] = None) -> int:
+    import torch
+    ns, off_d, off_q, need_d, _ = _batch_layout(ns, off_d, off_q)
+    _require_device("dv", dv, dtype=torch.float64, min_len=need_d, vector=True)
+    _require_device("ev", ev, dtype=torch.float64, device=dv.device, min_len=need_d - 1,
+                    vector=True)
     h = handle or default_handle()
-    ns, off_d, off_q = _host_i64(ns), _host_i64(off_d), _host_i64(off_q)
     choice = C.c_int32(-1)
     _check(lib().sep_tune_batched_f64(h.ptr, ns.shape[0], ns.ctypes.data, off_d.ctypes.data,
                                       off_q.ctypes.data, dv.data_ptr(), ev.data_ptr(),
@@ -410,12 +523,19 @@ def eig_backward(lam, Q, lam_bar, Q_bar, handle: Optional[Handle] = None):
 
     All device tensors, Float64; Q and Q_bar column-major n x n with the same ld."""
     import torch
-    h = handle or default_handle()
+    _require_device("lam", lam, dtype=torch.float64, vector=True)
     n = lam.shape[0]
-    _, _, ldq = _torch_colmajor(Q)
-    _, _, ldb = _torch_colmajor(Q_bar)
+    _require_device("lam_bar", lam_bar, dtype=torch.float64, device=lam.device, min_len=n,
+                    vector=True)
+    _require_device("Q", Q, dtype=torch.float64, device=lam.device)
+    _require_device("Q_bar", Q_bar, dtype=torch.float64, device=lam.device)
+    qm, qn, ldq = _torch_colmajor(Q)
+    bm, bn, ldb = _torch_colmajor(Q_bar)
+    if (qm, qn) != (n, n) or (bm, bn) != (n, n):
+        raise ValueError("Q and Q_bar must be n x n")
     if ldq != ldb:
         raise ValueError("Q and Q_bar must share a leading dimension")
+    h = handle or default_handle()
     d_bar = torch.empty(n, dtype=torch.float64, device=lam.device)
     e_bar = torch.empty(max(n - 1, 1), dtype=torch.float64, device=lam.device)
     _check(lib().sep_eig_backward_f64_dev(h.ptr, n, lam.data_ptr(), Q.data_ptr(), ldq,
@@ -427,8 +547,15 @@ def eig_backward(lam, Q, lam_bar, Q_bar, handle: Optional[Handle] = None):
 def eig_backward_batched(ns, off_d, off_q, lam, Q, lam_bar, Q_bar, handle: Optional[Handle] = None):
     """Batched backward over packed problems (layout as qr_batched_)."""
     import torch
+    ns, off_d, off_q, need_d, need_q = _batch_layout(ns, off_d, off_q)
+    _require_device("lam", lam, dtype=torch.float64, min_len=need_d, vector=True)
+    _require_device("lam_bar", lam_bar, dtype=torch.float64, device=lam.device, min_len=need_d,
+                    vector=True)
+    for name, t in (("Q", Q), ("Q_bar", Q_bar)):
+        _require_device(name, t, dtype=torch.float64, device=lam.device, min_len=need_q)
+        if not t.is_contiguous():
+            raise ValueError(f"{name} must be a contiguous packed buffer")
     h = handle or default_handle()
-    ns, off_d, off_q = _host_i64(ns), _host_i64(off_d), _host_i64(off_q)
     d_bar = torch.zeros_like(lam)
     e_bar = torch.zeros_like(lam)
     _check(lib().sep_eig_backward_batched_f64_dev(
@@ -449,8 +576,12 @@ def sytrd_(A, handle: Optional[Handle] = None):
     A torch device tensor runs in place on the device (d, e, tau are device tensors); a
     Fortran-ordered numpy array is copied to the device and back (numpy results)."""
     import torch
-    h = handle or default_handle()
     host = not _is_torch(A)
+    if not host:
+        _require_device("A", A, dtype=torch.float64)
+    elif A.dtype != np.float64:
+        raise ValueError("A must be square Float64")
+    h = handle or default_handle()
     Ad = to_device_colmajor(A) if host else A
     m, n, lda = _torch_colmajor(Ad)
     if m != n or Ad.dtype != torch.float64:
@@ -470,11 +601,17 @@ def sytrd_(A, handle: Optional[Handle] = None):
 def orgtr(A, tau, handle: Optional[Handle] = None):
     """The orthogonal factor Q = H(0) ... H(n-2) of sytrd_'s output (LAPACK dorgtr, 'L')."""
     import torch
-    h = handle or default_handle()
     host = not _is_torch(A)
+    if not host:
+        _require_device("A", A, dtype=torch.float64)
+        _require_device("tau", tau, dtype=torch.float64, device=A.device,
+                        min_len=max(A.shape[0] - 1, 0), vector=True)
+    h = handle or default_handle()
     Ad = to_device_colmajor(A) if host else A
     td = torch.from_numpy(np.ascontiguousarray(tau)).to(Ad.device) if host else tau
     m, n, lda = _torch_colmajor(Ad)
+    if m != n:
+        raise ValueError("A must be square")
     Q = torch.empty((n, n), dtype=torch.float64, device=Ad.device).t()
     _check(lib().sep_orgtr_f64_dev(h.ptr, n, Ad.data_ptr(), lda,
                                    td.data_ptr() if n > 1 else None, Q.data_ptr(), max(n, 1)))

@@ -22,8 +22,8 @@ LIB = os.path.join(HERE, "libsep_amd.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-SOURCES = ["capi.hip", "apply.hip", "apply_dpp.hip", "chase.hip", "batched.hip", "backward.hip",
-           "tridiag.hip"]
+SOURCES = ["capi.hip", "apply_dpp.hip", "chase.hip", "batched.hip", "backward.hip",
+           "tridiag.hip", "sort.hip"]
 
 CFLAGS = [
     "--offload-arch=gfx950",

@@ -4,10 +4,10 @@
 // Same product as bulk_wave_order_2x2_rmul! (src/wave.jl:62-162): A <- A * prod_L prod_i
 // G(i,L)^T, each rotation a' = c a + s b, b' = c b - s a on the column pair (i, i+1)
 // (Rotation2 rmul!, src/simple_givens.jl:53-64), and the same systolic organisation as the
-// SGPR-fed kernel in apply.hip (one row per lane, G waves of KG layers each, columns
-// handed from wave to wave through a double-buffered LDS ring, wave order t = i + 2L).
+// systolic organisation: one row per lane, G waves of KG layers each, columns handed from
+// wave to wave through a double-buffered LDS ring, wave order t = i + 2L.
 //
-// What differs is how the wave-uniform rotation data reach the FMA units:
+// How the wave-uniform rotation data reach the FMA units:
 //  * Coefficients are not scalar-loaded (scalar loads return out of order, so every use
 //    waits for all of them, exposing L2 latency each step).  Instead one 64-bit VGPR holds
 //    16 coefficients (2*KG per step, so one or two steps) in lanes 0..15 of each 16-lane
@@ -60,28 +60,58 @@ namespace {
 
 constexpr int kLanes = 64;
 
-// flag = 1 if any rotation of the grid is not normalised.
+// flag = 1 if any rotation of the grid is not normalised: |c^2 + s^2 - 1| > 2 eps(T).
+// The residual is evaluated as fma(c, c, fma(s, s, -1)), accurate to a fraction of eps,
+// and cos/sin pairs rounded to T (the benchmark's, and any correctly rounded pair) stay
+// below 0.71 eps.  Inside the bound the three-shear form realises a rotation that differs
+// from the given (c, s) by at most the residual per entry; outside it the grid takes the
+// five-instruction path, which applies (c, s) as given.
 template <typename T>
 __global__ void __launch_bounds__(256) k_check_norm(const T* __restrict__ cs, int64_t ld,
                                                     int n, int k, int* __restrict__ flag)
 {
     const int64_t total = (int64_t)(n - 1) * k;
+    constexpr T kTol = sizeof(T) == 8 ? T(0x1p-51) : T(0x1p-22);  // 2 eps
     int bad = 0;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
          q += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = q % (n - 1), L = q / (n - 1);
         const T c = cs[2 * (i + ld * L)], s = cs[2 * (i + ld * L) + 1];
-        bad |= !(fabs(c * c + s * s - T(1)) <= (sizeof(T) == 8 ? T(1e-14) : T(1e-6)));
+        bad |= !(fabs(fma(c, c, fma(s, s, T(-1)))) <= kTol);
     }
     if (__any(bad) && (threadIdx.x % kLanes) == 0) atomicOr(flag, 1);
 }
 
+// The reference's identity skip: bulk_wave_order_2x2_rmul! skips a fused block of four
+// (three at the edges) rotations only when all their s are zero (src/wave.jl:83,98,116,
+// 137,152), so a rotation with s = 0 and c != 1 is applied unless its whole block is
+// s-zero.  With 0-based (i, L), the block pairs layers Le = L & ~1 and Le + 1 and holds
+// rows b, b + 1 of layer Le and b - 1, b of layer Le + 1, where b = i & ~1 for even L and
+// (i + 1) & ~1 for odd L; rows outside 0 .. n-2 and layers past k are not part of it.
+template <typename T>
+__device__ __forceinline__ bool rot_skipped(const T* __restrict__ cs, int64_t ld, int n,
+                                            int64_t i, int L, int k)
+{
+    const int Le = L & ~1;
+    const int64_t b = (L & 1) ? ((i + 1) & ~(int64_t)1) : (i & ~(int64_t)1);
+    const int64_t rows[4] = {b, b + 1, b - 1, b};
+    const int lays[4] = {Le, Le, Le + 1, Le + 1};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (rows[q] < 0 || rows[q] > n - 2 || lays[q] >= k) continue;
+        if (cs[2 * (rows[q] + ld * lays[q]) + 1] != T(0)) return false;
+    }
+    return true;
+}
+
+// c < 0 of an applied rotation (a skipped or out-of-grid slot is the identity).
 template <typename T>
 __device__ __forceinline__ bool neg_c(const T* __restrict__ cs, int64_t ld, int n, int64_t i,
                                       int L, int k)
 {
     if (i < 0 || i > n - 2 || L >= k) return false;
-    return cs[2 * (i + ld * L)] < 0.0;
+    if (!(cs[2 * (i + ld * L)] < T(0))) return false;
+    return !rot_skipped<T>(cs, ld, n, i, L, k);
 }
 
 // Per column j: bit lr of mask[j] = the stored sign of column j is negative before the
@@ -148,7 +178,7 @@ __global__ void __launch_bounds__(256) k_colsigns(const T* __restrict__ cs, int6
 // PADU rows of padding before and enough after that every interval of every wave reads
 // inside the table).  Slot l of wave g at time t is rotation i = t - 2 (g KG + l) of layer
 // L0 + g KG + l; [2l] = p, [2l+1] = q (shear form) or c - 1, s (general form);
-// identity / out-of-grid slots hold (0, 0), which is the identity in both forms.
+// skipped (rot_skipped) and out-of-grid slots hold (0, 0), the identity in both forms.
 // Section g = G holds, for every interval bt, the first wave's 16-lane input-sign row
 // (sigm1 of the columns bt S + 1 .. bt S + S that enter the pass in that interval) at the
 // position a coefficient row of a (virtual) wave G would have, so the coefficient loaders
@@ -183,7 +213,7 @@ __global__ void __launch_bounds__(256) k_coef_table(T* __restrict__ tab,
         const int L = L0 + lr;
         const int64_t i = (int64_t)(u - S - PADU) - 2 * (int64_t)lr;
         T p = 0.0, qq = 0.0;
-        if (L < k && i >= 0 && i <= n - 2) {
+        if (L < k && i >= 0 && i <= n - 2 && !rot_skipped<T>(cs, ld, n, i, L, k)) {
             const T c = cs[2 * (i + ld * L)], s = cs[2 * (i + ld * L) + 1];
             if (general) {
                 p = c - 1.0;
@@ -709,7 +739,6 @@ int run_dpp(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T
     return SEP_OK;
 }
 
-using GeomDeep = Geom<4, 16, 8, 3>;   // 64 layers per pass, 4 waves per SIMD
 using GeomWide = Geom<8, 8, 16, 2>;   // 64 layers per pass, 2 waves per SIMD
 using GeomSmall = Geom<4, 8, 8, 3>;   // 32 layers per pass (short grids)
 
@@ -793,7 +822,7 @@ __global__ void __launch_bounds__(256) k_coef_r16(T* __restrict__ tab,
         const int L = L0 + lr;
         const int64_t i = u - (int64_t)q * (S - 1) - 2 * lr;
         T v = 0.0;
-        if (L < k && i >= 0 && i <= n - 2) {
+        if (L < k && i >= 0 && i <= n - 2 && !rot_skipped<T>(cs, ld, n, i, L, k)) {
             const T c = cs[2 * (i + ld * L)], s = cs[2 * (i + ld * L) + 1];
             if (general) {
                 v = (j & 1) ? s : c - 1.0;
@@ -1020,18 +1049,21 @@ int run_r16(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T
     return SEP_OK;
 }
 
-using G16Wide = G16<8, 2, 8>;   // 64 layers per pass: 2 waves of 4 groups of 8 layers
 using G16Deep = G16<4, 4, 8>;   // 64 layers per pass: 4 waves of 4 groups of 4 layers
-using G16Small = G16<4, 2, 8>;  // 32 layers per pass
 using G32Wide = G16<8, 4, 16, 32>;  // 32-row blocks: 4 waves of 2 groups of 8 layers
-using G32Deep = G16<4, 8, 8, 32>;   // 32-row blocks: 8 waves of 2 groups of 4 layers
 
 }  // namespace
 
 template <typename T>
-int bulk_apply_dpp(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T* A,
+int bulk_apply_dev(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T* A,
                    int64_t lda, const T* cs, int64_t ld_cs, int64_t k, bool rotations)
 {
+    if (m <= 0 || n <= 1 || k <= 0) return SEP_OK;
+    if (m > INT32_MAX || n > (INT32_MAX / 4)) {
+        set_error("bulk apply: m=%lld n=%lld beyond the 32-bit index range", (long long)m,
+                  (long long)n);
+        return SEP_ERR_ARG;
+    }
     // Measured kernel time per apply, m = n, k = 64 (ms):
     //            wide   deep   r16    r16deep r32    r32deep
     //   2000     0.304  0.317  0.200  0.192   0.227  0.259
@@ -1046,15 +1078,13 @@ int bulk_apply_dpp(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64
     // layers with two workgroups per CU 3.11 ms (KG 8, 4 waves) and 3.55 ms (KG 4, 8
     // waves); neighbour-to-neighbour interval counters in LDS instead of the workgroup
     // barrier 2.66 ms.
+    // SEP_APPLY forces one of the four geometries the size rule below chooses between
+    // (tests run every shape through each of them).
     const int geom = h ? h->apply_geom : 0;
     if (geom == 2) return run_dpp<T, GeomWide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if (geom == 1) return run_dpp<T, GeomDeep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
     if (geom == 5) return run_dpp<T, GeomSmall>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if (geom == 6) return run_r16<T, G16Wide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
     if (geom == 7) return run_r16<T, G16Deep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if (geom == 8) return run_r16<T, G16Small>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
     if (geom == 9) return run_r16<T, G32Wide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if (geom == 10) return run_r16<T, G32Deep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
     if (k <= 32) return run_dpp<T, GeomSmall>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
     if constexpr (sizeof(T) == 4) {
         // Float32, m = n, k = 64 (ms): 2000: wide 0.276, r16deep 0.164; 4096: 0.531, 0.309;
@@ -1069,9 +1099,9 @@ int bulk_apply_dpp(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64
     return run_dpp<T, GeomWide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
 }
 
-template int bulk_apply_dpp<double>(sep_handle*, hipStream_t, ApplyWork&, int64_t, int64_t,
+template int bulk_apply_dev<double>(sep_handle*, hipStream_t, ApplyWork&, int64_t, int64_t,
                                     double*, int64_t, const double*, int64_t, int64_t, bool);
-template int bulk_apply_dpp<float>(sep_handle*, hipStream_t, ApplyWork&, int64_t, int64_t,
+template int bulk_apply_dev<float>(sep_handle*, hipStream_t, ApplyWork&, int64_t, int64_t,
                                    float*, int64_t, const float*, int64_t, int64_t, bool);
 
 

@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(kLanes)
     else
         __builtin_amdgcn_s_setprio(1);
     chase::Problem<double> P{dv + pr.off_d, ev + pr.off_d, lt + pr.off_lt, grid + 2 * pr.off_g,
-                             pr.n, ML, (int64_t)(pr.n - 1), 0};
+                             pr.n, ML, (int64_t)(pr.n - 1)};
     const double tol = 2.220446049250313e-16;  // eps(Float64)
     if (in_lds) {
         double* ld = reinterpret_cast<double*>(dyn_lds);
@@ -113,7 +113,7 @@ __global__ void __launch_bounds__(kLanes)
             if (q + 1 < pr.n) le[q] = P.ev[q];
         }
         __syncthreads();
-        chase::Problem<double> PL{ld, le, ll, P.grid, pr.n, ML, (int64_t)(pr.n - 1), 0};
+        chase::Problem<double> PL{ld, le, ll, P.grid, pr.n, ML, (int64_t)(pr.n - 1)};
         chase::chase_block<double, kBatchedSolo>(PL, tol, 100 * (int64_t)pr.n, st, lane);
         __syncthreads();
         for (int q = lane; q < pr.n; q += kLanes) {

@@ -242,11 +242,9 @@ int sep_create(sep_handle** out, int device)
     h->device = device;
     h->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     if (const char* ap = std::getenv("SEP_APPLY")) {
-        h->force_sgpr_apply = std::strcmp(ap, "sgpr") == 0;
-        // Float64 apply geometry (unset: chosen by the number of rows)
+        // apply geometry (unset: chosen by the number of rows and layers)
         static const struct { const char* name; int geom; } kGeoms[] = {
-            {"deep", 1},    {"wide", 2},     {"small", 5}, {"r16", 6},
-            {"r16deep", 7}, {"r16small", 8}, {"r32", 9},   {"r32deep", 10}};
+            {"wide", 2}, {"small", 5}, {"r16deep", 7}, {"r32", 9}};
         for (const auto& g : kGeoms)
             if (std::strcmp(ap, g.name) == 0) h->apply_geom = g.geom;
     }
@@ -293,6 +291,7 @@ int sep_destroy(sep_handle* h)
     h->trd_vec.release();
     h->trd_part.release();
     h->trd_blk.release();
+    h->sort_rank.release();
     release_blas(h);
     for (auto e : h->tev) (void)hipEventDestroy(e);
     if (h->batch_graph) (void)hipGraphExecDestroy(h->batch_graph);
@@ -304,6 +303,9 @@ int sep_destroy(sep_handle* h)
     for (auto e : h->batch_ev)
         if (e) (void)hipEventDestroy(e);
     if (h->batch_pinned) (void)hipHostFree(h->batch_pinned);
+    if (h->solve_pinned) (void)hipHostFree(h->solve_pinned);
+    for (auto e : h->solve_ev)
+        if (e) (void)hipEventDestroy(e);
     if (h->ev_a) (void)hipEventDestroy(h->ev_a);
     if (h->ev_b) (void)hipEventDestroy(h->ev_b);
     if (h->chase_stream) (void)hipStreamDestroy(h->chase_stream);
@@ -499,6 +501,29 @@ int sep_eig_backward_batched_f64_dev(sep_handle* h, int64_t B, const int64_t* n,
     }
     SEP_HIP(hipSetDevice(h->device));
     return eig_backward_batched_f64(h, B, n, off_d, off_q, lam, Q, lam_bar, Q_bar, d_bar, e_bar);
+}
+
+int sep_sort_eigen_f64_dev(sep_handle* h, int64_t n, const double* lam, const double* Q,
+                           int64_t ldq, double* lam_out, double* Q_out, int64_t ldo,
+                           int64_t* perm)
+{
+    if (!h) {
+        set_error("sep_sort_eigen: null handle");
+        return SEP_ERR_ARG;
+    }
+    SEP_HIP(hipSetDevice(h->device));
+    return sort_eigen_dev<double>(h, n, lam, Q, ldq, lam_out, Q_out, ldo, perm);
+}
+
+int sep_sort_eigen_f32_dev(sep_handle* h, int64_t n, const float* lam, const float* Q,
+                           int64_t ldq, float* lam_out, float* Q_out, int64_t ldo, int64_t* perm)
+{
+    if (!h) {
+        set_error("sep_sort_eigen: null handle");
+        return SEP_ERR_ARG;
+    }
+    SEP_HIP(hipSetDevice(h->device));
+    return sort_eigen_dev<float>(h, n, lam, Q, ldq, lam_out, Q_out, ldo, perm);
 }
 
 int sep_sytrd_f64_dev(sep_handle* h, int64_t n, double* A, int64_t lda, double* d, double* e,

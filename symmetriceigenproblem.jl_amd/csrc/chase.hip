@@ -30,7 +30,7 @@ __global__ void __launch_bounds__(64) k_chase_block(T* dv, T* ev, int* lt, T* gr
     // the chase is the solve's critical path: its wave issues ahead of the apply of the
     // previous block running beside it
     __builtin_amdgcn_s_setprio(3);
-    chase::Problem<T> P{dv, ev, lt, grid, n, max_layers, (int64_t)(n - 1), 0};
+    chase::Problem<T> P{dv, ev, lt, grid, n, max_layers, (int64_t)(n - 1)};
     chase::chase_block(P, tol, maxiter, st, (int)threadIdx.x);
 }
 
@@ -43,15 +43,19 @@ int fill_identity(hipStream_t s, T* grid, int64_t pairs)
     return SEP_OK;
 }
 
-struct PinnedState {
-    int64_t* p = nullptr;
-    ~PinnedState()
-    {
-        if (p) (void)hipHostFree(p);
-    }
-};
-
 }  // namespace
+
+// The solve's pinned state words and its three events, created on the handle's first
+// solve and kept until sep_destroy.
+static int solve_resources(sep_handle* h)
+{
+    if (!h->solve_pinned)
+        SEP_HIP(hipHostMalloc((void**)&h->solve_pinned, sizeof(int64_t) * chase::ST_WORDS,
+                              hipHostMallocDefault));
+    for (auto& e : h->solve_ev)
+        if (!e) SEP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return SEP_OK;
+}
 
 template <typename T>
 int qr_algorithm_dev(sep_handle* h, int64_t n64, T* dv, T* ev, T* Q, int64_t ldq,
@@ -75,8 +79,8 @@ int qr_algorithm_dev(sep_handle* h, int64_t n64, T* dv, T* ev, T* Q, int64_t ldq
         if (int rc = h->grid[b].reserve(sizeof(T) * 2 * (size_t)pairs)) return rc;
     if (int rc = h->lt.reserve(sizeof(int) * (size_t)n)) return rc;
     if (int rc = h->state.reserve(sizeof(int64_t) * chase::ST_WORDS)) return rc;
-    PinnedState hs;
-    SEP_HIP(hipHostMalloc((void**)&hs.p, sizeof(int64_t) * chase::ST_WORDS, hipHostMallocDefault));
+    if (int rc = solve_resources(h)) return rc;
+    int64_t* const hst = h->solve_pinned;
     int64_t* dst = h->state.as<int64_t>();
 
     // the chase must see dv/ev as the caller's stream left them
@@ -87,14 +91,12 @@ int qr_algorithm_dev(sep_handle* h, int64_t n64, T* dv, T* ev, T* Q, int64_t ldq
     init[chase::ST_TO] = n;
     init[chase::ST_LBEA] = n;
     init[chase::ST_ITER] = 0;
-    std::memcpy(hs.p, init, sizeof(init));
-    SEP_HIP(hipMemcpyAsync(dst, hs.p, sizeof(init), hipMemcpyHostToDevice, cs));
+    // the previous solve's last read of the pinned words has completed (it synchronised)
+    std::memcpy(hst, init, sizeof(init));
+    SEP_HIP(hipMemcpyAsync(dst, hst, sizeof(init), hipMemcpyHostToDevice, cs));
 
-    hipEvent_t applied[2] = {nullptr, nullptr};
-    hipEvent_t chased = nullptr;
-    SEP_HIP(hipEventCreateWithFlags(&applied[0], hipEventDisableTiming));
-    SEP_HIP(hipEventCreateWithFlags(&applied[1], hipEventDisableTiming));
-    SEP_HIP(hipEventCreateWithFlags(&chased, hipEventDisableTiming));
+    hipEvent_t applied[2] = {h->solve_ev[0], h->solve_ev[1]};
+    hipEvent_t chased = h->solve_ev[2];
     bool applied_pending[2] = {false, false};
 
     int rc = SEP_OK;
@@ -115,7 +117,7 @@ int qr_algorithm_dev(sep_handle* h, int64_t n64, T* dv, T* ev, T* Q, int64_t ldq
             rc = SEP_ERR_HIP;
             break;
         }
-        if (hipMemcpyAsync(hs.p, dst, sizeof(int64_t) * chase::ST_WORDS, hipMemcpyDeviceToHost,
+        if (hipMemcpyAsync(hst, dst, sizeof(int64_t) * chase::ST_WORDS, hipMemcpyDeviceToHost,
                            cs) != hipSuccess ||
             hipEventRecord(chased, cs) != hipSuccess ||
             hipEventSynchronize(chased) != hipSuccess) {
@@ -123,7 +125,7 @@ int qr_algorithm_dev(sep_handle* h, int64_t n64, T* dv, T* ev, T* Q, int64_t ldq
             rc = SEP_ERR_HIP;
             break;
         }
-        const int64_t* st = hs.p;
+        const int64_t* st = hst;
         iter = st[chase::ST_ITER];
         if (st[chase::ST_EARLY]) {  // :180-182 -- return without applying this block
             conv = 0;
@@ -153,9 +155,6 @@ int qr_algorithm_dev(sep_handle* h, int64_t n64, T* dv, T* ev, T* Q, int64_t ldq
         set_error("qr_algorithm: %s", hipGetErrorString(se));
         rc = SEP_ERR_HIP;
     }
-    (void)hipEventDestroy(applied[0]);
-    (void)hipEventDestroy(applied[1]);
-    (void)hipEventDestroy(chased);
     if (rc) return rc;
     if (converged) *converged = conv;
     if (iter_out) *iter_out = iter;

@@ -382,11 +382,9 @@ __device__ __forceinline__ void capture_lane(float& oR, float& oD, float& oC, fl
     oS = mine ? s : oS;
 }
 
-// Problem view.  dv (n), ev (n-1), lt (n) and the (c,s) grid.  With tm == 0 the grid is
-// the reference's (n-1) x max_layers matrix of interleaved pairs (leading dimension ldg
-// pairs); with tm == max_layers > 0 it is time-major for the in-wave apply: rotation
-// (i, L) (0-based) at row i + 2L, column L of rows of tm pairs.  Indices in the member
-// functions are 1-based.
+// Problem view.  dv (n), ev (n-1), lt (n) and the (c,s) grid: the reference's
+// (n-1) x max_layers matrix of interleaved pairs (leading dimension ldg pairs).  Indices
+// in the member functions are 1-based.
 template <typename T>
 struct Problem {
     T* dv;
@@ -396,14 +394,12 @@ struct Problem {
     int n;
     int max_layers;
     int64_t ldg;
-    int tm;
 };
 
 // Pair index of rotation (i, layer) (both 1-based) in the grid.
 template <typename T>
 __device__ __forceinline__ int64_t grid_pair(const Problem<T>& P, int i, int layer)
 {
-    if (P.tm) return ((int64_t)(i - 1) + 2 * (int64_t)(layer - 1)) * P.tm + (layer - 1);
     return (int64_t)(i - 1) + P.ldg * (layer - 1);
 }
 

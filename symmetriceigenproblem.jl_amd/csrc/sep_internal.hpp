@@ -81,6 +81,9 @@ struct sep_handle {
     hipStream_t chase_stream = nullptr;  // chase stream (overlap with apply)
     bool own_stream = true;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    // single solves: pinned copy of the chase's state words, events applied[2] and chased
+    int64_t* solve_pinned = nullptr;
+    hipEvent_t solve_ev[3] = {nullptr, nullptr, nullptr};
     // scratch
     sep::DevBuf host_io_A, host_io_cs, host_io_d, host_io_e;
     sep::ApplyWork apply_work[2];
@@ -90,6 +93,7 @@ struct sep_handle {
     sep::DevBuf bwd_meta, bwd_M, bwd_part;
     // dense -> tridiagonal: reflector / update vectors and per-tile partial sums
     sep::DevBuf trd_vec, trd_part, trd_blk;
+    sep::DevBuf sort_rank;  // sorted output: rank of each eigenvalue
     void* blas = nullptr;  // rocblas_handle of the blocked orthogonal-factor GEMMs (lazy)
     std::vector<char> bwd_meta_host;
     void* bwd_meta_dev = nullptr;
@@ -110,7 +114,6 @@ struct sep_handle {
     std::vector<int64_t> tune_key;
     int tune_choice = -1;
     int force_kf = 0, force_r = 0;  // sep_set_batched_config (0: autotuned / default)
-    bool force_sgpr_apply = false;  // SEP_APPLY=sgpr selects the scalar-fed Float64 kernel
     int apply_geom = 0;  // apply geometry forced by SEP_APPLY (0: chosen by size)
     bool allow_odd = false;  // sep_set_odd_order: solves accept odd n
     // kernel timing (sep_set_timing): event pairs recorded around apply / prep kernels
@@ -131,14 +134,11 @@ inline void timing_pair(sep_handle* h, bool apply, int a, int b)
 
 namespace sep {
 
-// apply.hip.  `rotations` promises c^2 + s^2 = 1 for every pair (skips the check).
+// apply_dpp.hip (T = double or float).  `rotations` promises c^2 + s^2 = 1 for every
+// pair (skips the normalisation check).
 template <typename T>
 int bulk_apply_dev(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T* A,
                    int64_t lda, const T* cs, int64_t ld_cs, int64_t k, bool rotations = false);
-// apply_dpp.hip (DPP-fed kernels; T = double or float)
-template <typename T>
-int bulk_apply_dpp(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T* A,
-                   int64_t lda, const T* cs, int64_t ld_cs, int64_t k, bool rotations);
 
 // chase.hip
 template <typename T>
@@ -161,6 +161,11 @@ int eig_backward_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const i
                              const int64_t* off_q, const double* lam, const double* Q,
                              const double* lam_bar, const double* Q_bar, double* d_bar,
                              double* e_bar);
+
+// sort.hip
+template <typename T>
+int sort_eigen_dev(sep_handle* h, int64_t n, const T* lam, const T* Q, int64_t ldq, T* lam_out,
+                   T* Q_out, int64_t ldo, int64_t* perm);
 
 // tridiag.hip
 int sytrd_f64(sep_handle* h, int64_t n, double* A, int64_t lda, double* d, double* e,

@@ -107,7 +107,7 @@ def test_full_size_16384_rows_subset(sep, handle, dtype):
     assert rel <= (1e-13 if dtype == np.float64 else 1e-4)
 
 
-GEOMETRIES = ["wide", "deep", "small", "r16", "r16deep", "r16small", "r32", "r32deep", "sgpr"]
+GEOMETRIES = ["wide", "small", "r16deep", "r32"]
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
@@ -142,3 +142,77 @@ def test_every_geometry_matches_oracle(sep, monkeypatch, geom, dtype):
         assert np.array_equal(out, A), geom
     finally:
         h.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_config2_n2000_whole_matrix(sep, handle, dtype):
+    """The reference's published shape, bench(T, 2000, 2000, 64): the whole matrix against
+    the oracle's layer-major product."""
+    m = n = 2000
+    k = 64
+    A, g = _case(m, n, k, dtype, seed=2000)
+    ref = A.copy(order="F")
+    oracle.bulk_rmul_seq(ref, g)
+    out = A.copy(order="F")
+    sep.bulk_wave_order_2x2_rmul_(out, g, handle=handle)
+    assert _relerr(out, ref) <= TOL[dtype], _relerr(out, ref)
+
+
+def _s_zero_grid(n, k, dtype, seed, cvals):
+    """Proper rotations with 60% of the slots replaced by (c, 0), c drawn from cvals."""
+    r = np.random.default_rng(seed)
+    th = r.random((k, n - 1)) * 2 * np.pi
+    g = np.stack([np.cos(th), np.sin(th)], -1)
+    mask = r.random((k, n - 1)) < 0.6
+    c = r.choice(cvals, size=(k, n - 1))
+    g[mask, 0] = c[mask]
+    g[mask, 1] = 0.0
+    return np.ascontiguousarray(g.astype(dtype))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("geom", GEOMETRIES)
+def test_s_zero_entries_follow_the_reference_block_skip(sep, monkeypatch, geom, dtype):
+    """Entries (c, 0) with c != 1: the reference applies them (scaling or negating their
+    columns) unless their whole fused block is s-zero (src/wave.jl:83,98,116,137,152).
+    Grids with c in {-1, 1} are rotations (three-shear path, the sign goes into the column
+    sign tracking); with 0.5 and 2 they are not (five-instruction path).  Checked against
+    the oracle, whose rule is pinned bit for bit to the wave restatement."""
+    monkeypatch.setenv("SEP_APPLY", geom)
+    h = sep.Handle(0)
+    try:
+        for cvals in ([-1.0, 1.0], [-1.0, 0.5, 1.0, 2.0]):
+            for m, n, k in ((70, 150, 64), (33, 71, 130), (64, 10, 8)):
+                A = sep.inputs.gaussian_colmajor(m, n, m + n, dtype)
+                g = _s_zero_grid(n, k, dtype, seed=n + k, cvals=cvals)
+                ref = A.copy(order="F")
+                oracle.bulk_rmul_seq(ref, g)
+                out = A.copy(order="F")
+                sep.bulk_wave_order_2x2_rmul_(out, g, handle=h)
+                assert _relerr(out, ref) <= TOL[dtype], (geom, cvals, m, n, k,
+                                                         _relerr(out, ref))
+    finally:
+        h.close()
+
+
+@pytest.mark.parametrize("dtype,delta", [(np.float64, 9e-15), (np.float64, -9e-15),
+                                         (np.float32, 5e-7), (np.float32, -5e-7)])
+def test_near_normalised_grid_outside_the_bound(sep, handle, dtype, delta):
+    """Pairs with c^2 + s^2 = 1 + delta, delta of one sign for every rotation and beyond the
+    normalisation bound (2 eps): the three-shear form would realise a determinant-one
+    rotation and drift by ~k delta, so these must take the path that applies (c, s) as
+    given, and match the oracle to the usual tolerance."""
+    m, n, k = 64, 400, 64
+    r = np.random.default_rng(17)
+    th = r.random((k, n - 1)) * 2 * np.pi
+    f = np.sqrt(1.0 + delta)
+    g = np.ascontiguousarray(np.stack([np.cos(th) * f, np.sin(th) * f], -1).astype(dtype))
+    c, s = g[..., 0].astype(np.float64), g[..., 1].astype(np.float64)
+    res = np.abs(c * c + s * s - 1)
+    assert res.min() > 2 * np.finfo(dtype).eps  # every pair is outside the bound
+    A = sep.inputs.gaussian_colmajor(m, n, 3, dtype)
+    ref = A.copy(order="F")
+    oracle.bulk_rmul_seq(ref, g)
+    out = A.copy(order="F")
+    sep.bulk_wave_order_2x2_rmul_(out, g, handle=handle)
+    assert _relerr(out, ref) <= TOL[dtype], _relerr(out, ref)

@@ -176,3 +176,36 @@ def test_backward_errors(sep, handle):
     with pytest.raises(ValueError):
         sep.eig_backward_batched(np.array([4, 0]), np.array([0, 4]), np.array([0, 16]), z, z,
                                  z, z, handle=handle)
+
+
+def test_backward_batched_1024_problems(sep, handle):
+    """The differentiated-solve batch at its stated size, as timed: 1024 problems of even
+    order in [64, 256], solved by the batched solver (eigenvalues in deflation order), N(0,1)
+    cotangents; every problem against the dense formula."""
+    import torch
+    Bb = 1024
+    rb = np.random.default_rng(11)
+    ns = rb.integers(64, 257, size=Bb).astype(np.int64)
+    ns += ns % 2
+    d, e, off_d, off_q = sep.inputs.batched_problems(ns, seed=13)
+    totq = int(off_q[-1] + ns[-1] ** 2)
+    print(f"[backward] batched B={Bb} sum n^3={int((ns.astype(np.float64) ** 3).sum()):.3e}",
+          flush=True)
+    dv = torch.from_numpy(d.copy()).cuda()
+    ev = torch.from_numpy(e.copy()).cuda()
+    Q = torch.zeros(totq, dtype=torch.float64, device="cuda")
+    for p, n in enumerate(ns):
+        Q[off_q[p]:off_q[p] + n * n].view(n, n).fill_diagonal_(1.0)
+    conv, _ = sep.qr_batched_(ns, off_d, off_q, dv, ev, Q, handle=handle)
+    assert conv.all()
+    lbar = rb.standard_normal(dv.shape[0])
+    qbar = rb.standard_normal(totq)
+    db, eb = sep.eig_backward_batched(ns, off_d, off_q, dv, Q, _dev(lbar), _dev(qbar),
+                                      handle=handle)
+    lam, Qh, db, eb = dv.cpu().numpy(), Q.cpu().numpy(), db.cpu().numpy(), eb.cpu().numpy()
+    for p, n in enumerate(ns):
+        o, q = off_d[p], off_q[p]
+        Qp = Qh[q:q + n * n].reshape(n, n).T
+        Gp = qbar[q:q + n * n].reshape(n, n).T
+        rd, re = oracle.eig_backward_dense(lam[o:o + n], Qp, lbar[o:o + n], Gp)
+        _check(db[o:o + n], eb[o:o + n - 1], rd, re, 1e-11)

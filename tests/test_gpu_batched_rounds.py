@@ -215,3 +215,41 @@ def test_rounds_large_orders(sep, handle, ns):
     ns, d, e, off_d, off_q = _batch(sep, ns, seed=int(sum(ns)))
     conv, iters, gd, ge, gQ = _solve(sep, handle, ns, d, e, off_d, off_q, max_layers=32)
     _check_bitwise(ns, d, e, off_d, off_q, 32, conv, iters, gd, ge, gQ)
+
+
+def _tridiag_residuals(d, e, Q, lam):
+    """||T Q - Q diag(lam)||_F for the tridiagonal T(d, e), in O(n^2)."""
+    TQ = d[:, None] * Q
+    TQ[:-1] += e[:, None] * Q[1:]
+    TQ[1:] += e[:, None] * Q[:-1]
+    return np.linalg.norm(TQ - Q * lam[None, :])
+
+
+def test_full_batch_4096_problems(sep, handle):
+    """The batched workload at its stated size: 4096 problems of even order in [32, 512]
+    (seed 5, as timed), autotuned configuration.  Every problem: converged, eigenvalues
+    against LAPACK to 1e-12, ||T Q - Q Lam|| <= 1e-13 ||T||, ||Q^T Q - I|| <= 1e-13 n.
+    Then the same batch with the default block length, 32 problems (the largest and 31
+    spread over the batch) bit for bit against the oracle."""
+    ns = sep.inputs.batched_orders(4096)
+    print(f"[batched] full batch B={len(ns)} sum n^2={int((ns * ns).sum())}", flush=True)
+    ns, d, e, off_d, off_q = _batch(sep, ns, seed=5)
+    conv, iters, gd, ge, gQ = _solve(sep, handle, ns, d, e, off_d, off_q, 0, tune=True)
+    assert conv.all()
+    worst = [0.0, 0.0, 0.0]
+    for p in range(len(ns)):
+        n, o = int(ns[p]), off_d[p]
+        dd, ee, lam = d[o:o + n], e[o:o + n - 1], gd[o:o + n]
+        ref = sl.eigvalsh_tridiagonal(dd, ee)
+        Qg = gQ[off_q[p]:off_q[p] + n * n].reshape(n, n).T
+        tn = np.sqrt((dd * dd).sum() + 2 * (ee * ee).sum())
+        worst[0] = max(worst[0], np.abs(np.sort(lam) - ref).max() / np.abs(ref).max())
+        worst[1] = max(worst[1], _tridiag_residuals(dd, ee, Qg, lam) / tn)
+        worst[2] = max(worst[2], np.linalg.norm(Qg.T @ Qg - np.eye(n)) / n)
+    print(f"[batched] worst eig {worst[0]:.2e} residual {worst[1]:.2e} orth/n {worst[2]:.2e}",
+          flush=True)
+    assert worst[0] <= 1e-12 and worst[1] <= 1e-13 and worst[2] <= 1e-13
+    conv, iters, gd, ge, gQ = _solve(sep, handle, ns, d, e, off_d, off_q, 16)
+    which = sorted({int(np.argmax(ns))} | set(range(0, len(ns), 133)))
+    assert len(which) >= 32
+    _check_bitwise(ns, d, e, off_d, off_q, 16, conv, iters, gd, ge, gQ, which=which)

@@ -235,3 +235,69 @@ def test_extreme_magnitudes_bitwise(sep, handle, scale_exp):
     ref = np.ldexp(sl.eigvalsh_tridiagonal(np.ldexp(d, -scale_exp), np.ldexp(e, -scale_exp)),
                    scale_exp)
     assert np.abs(np.sort(gd) - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_eigen_decomp_sorted_output(sep, handle):
+    """eigen_decomp_ (src/bulge_chasing.jl:290-299) on device tensors, and its sorted form:
+    eigenvalues ascending (LAPACK's order) with the columns of Q permuted to match, so the
+    residual still holds column by column; perm maps sorted positions to the deflation
+    order."""
+    import torch
+    n = 300
+    d, e = sep.inputs.tridiag_linear(n, seed=8)
+    A = _dense(d, e)
+    H = sep.SymTridiagonal(torch.from_numpy(d.copy()).cuda(), torch.from_numpy(e.copy()).cuda())
+    lam, Q = sep.eigen_decomp_(H, handle=handle)
+    lam_h, Q_h = lam.cpu().numpy(), sep.to_host_colmajor(Q)
+    ls, Qs, perm = sep.sort_eigen(lam, Q, handle=handle)
+    ls, Qs, perm = ls.cpu().numpy(), sep.to_host_colmajor(Qs), perm.cpu().numpy()
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    assert np.array_equal(ls, lam_h[perm]) and np.array_equal(Qs, Q_h[:, perm])
+    assert np.all(np.diff(ls) >= 0)
+    assert np.abs(ls - sl.eigvalsh_tridiagonal(d, e)).max() <= 1e-12 * n
+    assert np.linalg.norm(A @ Qs - Qs * ls) <= 1e-13 * np.linalg.norm(A)
+    H2 = sep.SymTridiagonal(torch.from_numpy(d.copy()).cuda(), torch.from_numpy(e.copy()).cuda())
+    lam2, Q2 = sep.eigen_decomp_(H2, handle=handle, sort=True)
+    assert np.array_equal(lam2.cpu().numpy(), ls)
+    assert np.array_equal(sep.to_host_colmajor(Q2), Qs)
+
+
+def test_sort_eigen_ties_nans_and_float32(sep, handle):
+    """The order is total: ties keep index order, -0 sorts before +0, NaNs go last (with
+    the sign bit set: first); eigenvalues only (Q = None) and Float32 work the same."""
+    import torch
+    v = np.array([3.0, -1.0, np.nan, 2.0, -1.0, 0.0, -0.0, np.inf, -np.inf, 2.0])
+    for dt in (torch.float64, torch.float32):
+        lam = torch.tensor(v, dtype=dt).cuda()
+        ls, Qs, perm = sep.sort_eigen(lam, None, handle=handle)
+        assert Qs is None
+        assert perm.cpu().tolist() == [8, 1, 4, 6, 5, 3, 9, 0, 7, 2]
+        got = ls.cpu().numpy()
+        assert np.array_equal(got[:-1], v[[8, 1, 4, 6, 5, 3, 9, 0, 7]].astype(got.dtype))
+        assert np.signbit(got[3]) and not np.signbit(got[4]) and np.isnan(got[-1])
+    r = np.random.default_rng(2)
+    lam = torch.from_numpy(r.standard_normal(1000).astype(np.float32)).cuda()
+    Q = sep.to_device_colmajor(np.asfortranarray(r.standard_normal((1000, 1000)).astype(np.float32)))
+    ls, Qs, perm = sep.sort_eigen(lam, Q, handle=handle)
+    p = perm.cpu().numpy()
+    assert np.array_equal(p, np.argsort(lam.cpu().numpy(), kind="stable"))
+    assert np.array_equal(sep.to_host_colmajor(Qs), sep.to_host_colmajor(Q)[:, p])
+
+
+def test_eigen_decomp_reports_non_convergence(sep, handle, caplog):
+    """The reference logs converged / iter (@info, src/bulge_chasing.jl:296); here they go
+    to the "sep_amd" logger, and a solve that stopped at maxiter also warns."""
+    import logging
+    n = 80
+    d, e = sep.inputs.tridiag_linear(n, seed=4)
+    H = sep.SymTridiagonal(d.copy(), e.copy())
+    with caplog.at_level(logging.INFO, logger="sep_amd"):
+        with pytest.warns(RuntimeWarning, match="did not converge"):
+            sep.eigen_decomp_(H, handle=handle, max_layers=16, maxiter=10)
+    assert "converged=False iter=11" in caplog.text
+    caplog.clear()
+    H = sep.SymTridiagonal(d.copy(), e.copy())
+    with caplog.at_level(logging.INFO, logger="sep_amd"):
+        lam, Q = sep.eigen_decomp_(H, handle=handle)
+    assert "converged=True" in caplog.text
+    assert np.linalg.norm(_dense(d, e) @ Q - Q * lam) <= 1e-13 * np.linalg.norm(_dense(d, e))

@@ -394,3 +394,76 @@ def test_fd_directional_matches_dense_gradient():
     gd2 = gd.copy()
     gd2[7] += 1e-3
     assert abs(gd2 @ vd + ge @ ve - fd) > 1e-6 * (abs(g) + np.sqrt(gd @ gd + ge @ ge))
+
+
+# ---------------------------------------------------------------- identity skip rule
+
+def _apply_rule(A, g, rule):
+    """Layer-major product with one of three identity rules, in plain numpy: 'none' applies
+    every rotation, 'each' skips every rotation with s == 0, 'block' is the oracle."""
+    A = A.copy(order="F")
+    k, nm1, _ = g.shape
+    for L in range(k):
+        for i in range(nm1):
+            c, s = g[L, i]
+            if rule == "each" and s == 0:
+                continue
+            a, b = A[:, i].copy(), A[:, i + 1].copy()
+            A[:, i], A[:, i + 1] = a * c + b * s, a * -s + b * c
+    return A
+
+
+def _s_zero_grid(n, k, seed, frac=0.6):
+    r = np.random.default_rng(seed)
+    th = r.random((k, n - 1)) * 2 * np.pi
+    g = np.ascontiguousarray(np.stack([np.cos(th), np.sin(th)], -1))
+    mask = r.random((k, n - 1)) < frac
+    cval = r.choice([-1.0, 0.5, 1.0, 2.0], size=(k, n - 1))
+    g[mask, 0] = cval[mask]
+    g[mask, 1] = 0.0
+    return g
+
+
+@pytest.mark.parametrize("n,k", [(4, 2), (10, 4), (12, 10), (34, 32), (100, 98), (2000, 64)])
+def test_block_skip_rule_equals_wave_order(n, k):
+    """Rotations with s == 0 and c in {-1, 0.5, 1, 2}: the sequential oracle's fused-block
+    skip gives the wave restatement's result (src/wave.jl:83,98,116,137,152) bit for bit,
+    serial and row-split."""
+    g = _s_zero_grid(n, k, seed=n + k)
+    A = np.asfortranarray(np.random.default_rng(n).standard_normal((5, n)))
+    a1, a2, a3 = A.copy(order="F"), A.copy(order="F"), A.copy(order="F")
+    oracle.bulk_rmul_seq(a1, g)
+    oracle.bulk_wave(a2, g)
+    oracle.bulk_wave(a3, g, threads=3)
+    assert np.array_equal(a1, a2) and np.array_equal(a1, a3)
+
+
+def test_block_skip_rule_is_neither_skip_all_nor_skip_none():
+    """The rule is pinned against both plausible misreadings: skipping every s == 0
+    rotation, and applying every rotation, each differ from the wave order on a grid with
+    (-1, 0) and (0.5, 0) entries; a fully s-zero block holding (-1, 0) changes nothing."""
+    n, k = 10, 4
+    g = _s_zero_grid(n, k, seed=7)
+    A = np.asfortranarray(np.random.default_rng(8).standard_normal((5, n)))
+    w = A.copy(order="F")
+    oracle.bulk_wave(w, g)
+    seq = A.copy(order="F")
+    oracle.bulk_rmul_seq(seq, g)
+    assert np.array_equal(seq, w)
+    assert np.abs(_apply_rule(A, g, "each") - w).max() > 1e-3
+    assert np.abs(_apply_rule(A, g, "none") - w).max() > 1e-3
+    # block (start_col = 2, layers 1, 2): rows 3, 4 of layer 1 and rows 2, 3 of layer 2
+    # (1-based); only it is s-zero, with c = -1 everywhere in it
+    g = oracle.identity_grid(n, 2)
+    th = np.random.default_rng(9).random((2, n - 1)) * 2 * np.pi
+    g[..., 0], g[..., 1] = np.cos(th), np.sin(th)
+    for (row, lay) in ((3, 1), (4, 1), (2, 2), (3, 2)):
+        g[lay - 1, row - 1] = (-1.0, 0.0)
+    ref = A.copy(order="F")
+    oracle.bulk_wave(ref, g)
+    g2 = g.copy()
+    for (row, lay) in ((3, 1), (4, 1), (2, 2), (3, 2)):
+        g2[lay - 1, row - 1] = (1.0, 0.0)
+    out = A.copy(order="F")
+    oracle.bulk_rmul_seq(out, g2)
+    assert np.array_equal(ref, out)
