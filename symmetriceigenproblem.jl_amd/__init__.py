@@ -52,6 +52,7 @@ EXPORTS = (
     "sep_create", "sep_destroy", "sep_last_error", "sep_version", "sep_set_stream",
     "sep_synchronize", "sep_set_timing", "sep_get_timing",
     "sep_bulk_rmul_f64", "sep_bulk_rmul_f32", "sep_bulk_rmul_f64_dev", "sep_bulk_rmul_f32_dev",
+    "sep_bulk_rmul_f64_dev_noalloc",
     "sep_qr_algorithm_f64", "sep_qr_algorithm_f32", "sep_qr_algorithm_f64_dev",
     "sep_qr_algorithm_f32_dev",
     "sep_qr_batched_f64_dev", "sep_tune_batched_f64", "sep_set_batched_config",
@@ -107,6 +108,7 @@ def _declare(L: C.CDLL) -> None:
             f.argtypes = [_P, _I64, _I64, _P, _I64, _P, _I64]
             f = getattr(L, f"sep_qr_algorithm_{suf}{dev}")
             f.argtypes = [_P, _I64, _P, _P, _P, _I64, _I64, ct, _I64, _P, _P, _P]
+    L.sep_bulk_rmul_f64_dev_noalloc.argtypes = [_P, _I64, _I64, _P, _I64, _P, _I64]
     L.sep_qr_batched_f64_dev.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _I64, _P, _P]
     L.sep_tune_batched_f64.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P]
     L.sep_set_batched_config.argtypes = [_P, C.c_int32, C.c_int32]
@@ -142,10 +144,22 @@ def _check(rc: int, allow_not_converged: bool = False) -> int:
     raise SepError(rc, msg)
 
 
+def _init_torch_runtime() -> None:
+    """Initialise torch's GPU runtime before the library's.  torch carries its own copy of
+    the HIP runtime beside the one libsep_amd.so links; with both in one process, torch's
+    must come up first (the other order leaves torch reporting no GPU)."""
+    try:
+        import torch
+    except ImportError:
+        return
+    torch.cuda.is_available()
+
+
 class Handle:
     """Owns a sep_handle (device buffers, chase/apply streams, tuning table)."""
 
     def __init__(self, device: int = 0):
+        _init_torch_runtime()
         self._h = _P()
         _check(lib().sep_create(C.byref(self._h), int(device)))
         self.device = device

@@ -51,6 +51,7 @@
 //    five-instruction path, t = a; a += (c-1) a; a += s b; b += (c-1) b; b -= s t, chosen
 //    on the device by a normalisation check, so the entry point accepts any (c, s) as the
 //    reference's kernels do.
+#include <mutex>
 #include <type_traits>
 
 #include "sep_internal.hpp"
@@ -80,6 +81,33 @@ __global__ void __launch_bounds__(256) k_check_norm(const T* __restrict__ cs, in
         bad |= !(fabs(fma(c, c, fma(s, s, T(-1)))) <= kTol);
     }
     if (__any(bad) && (threadIdx.x % kLanes) == 0) atomicOr(flag, 1);
+}
+
+// Clears the normalisation flag.  A kernel rather than hipMemsetAsync, so that the launch
+// sequence holds no memset node when a caller captures it into a graph.
+__global__ void __launch_bounds__(64) k_zero_flag(int* __restrict__ flag)
+{
+    if (threadIdx.x == 0) flag[0] = 0;
+}
+
+// Device memory of one apply call: the coefficient table and the auxiliary words (column
+// sign masks, output signs, normalisation flag).
+template <typename T>
+struct ApplyMem {
+    T* table;
+    uint64_t* mask;
+    T* sigm1;
+    int* flag;
+};
+
+// The auxiliary words of an n-column apply, laid out from `aux` (256-byte aligned).
+inline size_t aux_bytes(int64_t n, size_t tsize) { return (8 + tsize) * (size_t)n + 256; }
+template <typename T>
+ApplyMem<T> apply_mem(void* table, void* aux, int64_t n)
+{
+    uint64_t* mask = static_cast<uint64_t*>(aux);
+    T* sigm1 = reinterpret_cast<T*>(mask + n);
+    return ApplyMem<T>{static_cast<T*>(table), mask, sigm1, reinterpret_cast<int*>(sigm1 + n)};
 }
 
 // The reference's identity skip: bulk_wave_order_2x2_rmul! skips a fused block of four
@@ -692,23 +720,35 @@ __global__ void __launch_bounds__(kLanes * Gm::G)
 #undef SEP_WAVE
 }
 
+// Interval counts of the systolic geometry for n columns.
+template <class Gm>
+struct DppDims {
+    int NI, PADU, Up;
+    explicit DppDims(int64_t n)
+    {
+        const int T_out = (int)(n + 2 * Gm::Kp - 2);
+        NI = Gm::G + (T_out + Gm::S - 1) / Gm::S + 1;
+        PADU = (Gm::G - 1) * (Gm::S - 1);
+        Up = NI * Gm::S + 2 * Gm::S + PADU;
+    }
+    size_t table_elems() const { return (size_t)(Gm::G + 1) * Up * Gm::NC + 64; }
+};
+
+// th: the handle whose timing counters record the launches (may be null).
 template <typename T, class Gm>
-int run_dpp(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T* A,
+int run_dpp(sep_handle* th, hipStream_t st, const ApplyMem<T>& mem, int64_t m, int64_t n, T* A,
             int64_t lda, const T* cs, int64_t ld_cs, int64_t k, bool rotations)
 {
     constexpr int G = Gm::G, S = Gm::S, KG = Gm::KG, Kp = Gm::Kp;
-    const int T_out = (int)(n + 2 * Kp - 2);
-    const int NI = G + (T_out + S - 1) / S + 1;
-    const int PADU = (G - 1) * (S - 1);
-    const int Up = NI * S + 2 * S + PADU;
-    if (int rc = w.table.reserve(sizeof(T) * ((size_t)(G + 1) * Up * Gm::NC + 64))) return rc;
-    if (int rc = w.aux.reserve(sizeof(uint64_t) * (size_t)n + sizeof(T) * (size_t)n + 256))
-        return rc;
-    uint64_t* mask = w.aux.as<uint64_t>();
-    T* sigm1 = reinterpret_cast<T*>(mask + n);
-    int* flag = reinterpret_cast<int*>(sigm1 + n);
+    sep_handle* h = th;
+    const DppDims<Gm> dd(n);
+    const int NI = dd.NI, PADU = dd.PADU, Up = dd.Up;
+    uint64_t* mask = mem.mask;
+    T* sigm1 = mem.sigm1;
+    int* flag = mem.flag;
     const int tp0 = timing_mark(h, st);
-    SEP_HIP(hipMemsetAsync(flag, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(64), 0, st, flag);
+    SEP_CHECK_LAUNCH();
     if (!rotations) {
         const int64_t tot = (n - 1) * k;
         const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 2048);
@@ -725,13 +765,13 @@ int run_dpp(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T
         const int64_t total = (int64_t)G * Up * KG + (int64_t)NI * 16;
         const int tblocks = (int)std::min<int64_t>((total + 255) / 256, 8192);
         hipLaunchKernelGGL((k_coef_table<T, KG, G, S>), dim3(tblocks), dim3(256), 0, st,
-                           w.table.as<T>(), cs, ld_cs, (int)n, (int)k, (int)L0, Up, PADU,
-                           NI, flag, mask, sigm1);
+                           mem.table, cs, ld_cs, (int)n, (int)k, (int)L0, Up, PADU, NI, flag,
+                           mask, sigm1);
         SEP_CHECK_LAUNCH();
         const int ta = timing_mark(h, st);
         timing_pair(h, false, tp_prev, ta);
         hipLaunchKernelGGL((k_apply_dpp<T, Gm>), dim3((unsigned)rb), dim3(kLanes * G), 0, st, A,
-                           lda, (int)m, (int)n, w.table.as<T>(), sigm1, flag, Up, PADU, NI);
+                           lda, (int)m, (int)n, mem.table, sigm1, flag, Up, PADU, NI);
         SEP_CHECK_LAUNCH();
         tp_prev = timing_mark(h, st);
         timing_pair(h, true, ta, tp_prev);
@@ -1003,23 +1043,34 @@ __global__ void __launch_bounds__(kLanes * Gm::NW)
 }
 // (R16_END)
 
+// Interval counts of a 16- or 32-row geometry for n columns.
+template <class Gm>
+struct R16Dims {
+    int NB, NBt;
+    explicit R16Dims(int64_t n)
+    {
+        NB = (int)((n + Gm::DL + Gm::S - 1) / Gm::S);
+        NB = (NB + Gm::U - 1) / Gm::U * Gm::U;  // whole unrolled loop bodies
+        NBt = NB + 2;
+    }
+    size_t table_elems() const { return (size_t)Gm::NW * NBt * Gm::NR * kLanes; }
+};
+
 template <typename T, class Gm>
-int run_r16(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T* A,
+int run_r16(sep_handle* th, hipStream_t st, const ApplyMem<T>& mem, int64_t m, int64_t n, T* A,
             int64_t lda, const T* cs, int64_t ld_cs, int64_t k, bool rotations)
 {
-    constexpr int S = Gm::S, Kp = Gm::Kp;
-    int NB = (int)((n + Gm::DL + S - 1) / S);
-    NB = (NB + Gm::U - 1) / Gm::U * Gm::U;  // whole unrolled loop bodies
-    const int NBt = NB + 2;
-    const size_t tab_elems = (size_t)Gm::NW * NBt * Gm::NR * kLanes;
-    if (int rc = w.table.reserve(sizeof(T) * tab_elems)) return rc;
-    if (int rc = w.aux.reserve(sizeof(uint64_t) * (size_t)n + sizeof(T) * (size_t)n + 256))
-        return rc;
-    uint64_t* mask = w.aux.as<uint64_t>();
-    T* sigm1 = reinterpret_cast<T*>(mask + n);
-    int* flag = reinterpret_cast<int*>(sigm1 + n);
+    constexpr int Kp = Gm::Kp;
+    sep_handle* h = th;
+    const R16Dims<Gm> rd(n);
+    const int NB = rd.NB, NBt = rd.NBt;
+    const size_t tab_elems = rd.table_elems();
+    uint64_t* mask = mem.mask;
+    T* sigm1 = mem.sigm1;
+    int* flag = mem.flag;
     const int tp0 = timing_mark(h, st);
-    SEP_HIP(hipMemsetAsync(flag, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(64), 0, st, flag);
+    SEP_CHECK_LAUNCH();
     if (!rotations) {
         const int64_t tot = (n - 1) * k;
         const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 2048);
@@ -1034,14 +1085,13 @@ int run_r16(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T
                            ld_cs, (int)n, (int)k, (int)L0, Kp, 0, flag, mask, sigm1);
         SEP_CHECK_LAUNCH();
         const int tblocks = (int)std::min<int64_t>(((int64_t)tab_elems + 255) / 256, 8192);
-        hipLaunchKernelGGL((k_coef_r16<T, Gm>), dim3(tblocks), dim3(256), 0, st,
-                           w.table.as<T>(), cs, ld_cs, (int)n, (int)k, (int)L0, NBt, flag,
-                           mask);
+        hipLaunchKernelGGL((k_coef_r16<T, Gm>), dim3(tblocks), dim3(256), 0, st, mem.table, cs,
+                           ld_cs, (int)n, (int)k, (int)L0, NBt, flag, mask);
         SEP_CHECK_LAUNCH();
         const int ta = timing_mark(h, st);
         timing_pair(h, false, tp_prev, ta);
         hipLaunchKernelGGL((k_apply_r16<T, Gm>), dim3((unsigned)wgs), dim3(kLanes * Gm::NW), 0, st,
-                           A, lda, (int)m, (int)n, w.table.as<T>(), flag, sigm1, NB, NBt);
+                           A, lda, (int)m, (int)n, mem.table, flag, sigm1, NB, NBt);
         SEP_CHECK_LAUNCH();
         tp_prev = timing_mark(h, st);
         timing_pair(h, true, ta, tp_prev);
@@ -1052,6 +1102,75 @@ int run_r16(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T
 using G16Deep = G16<4, 4, 8>;   // 64 layers per pass: 4 waves of 4 groups of 4 layers
 using G32Wide = G16<8, 4, 16, 32>;  // 32-row blocks: 4 waves of 2 groups of 8 layers
 
+// The four geometries (SEP_APPLY names them wide, small, r16deep, r32).
+enum ApplyGeomId { kGeomWide = 2, kGeomSmall = 5, kGeomR16Deep = 7, kGeomR32 = 9 };
+
+// Measured kernel time per apply, m = n, k = 64 (ms):
+//            wide   deep   r16    r16deep r32    r32deep
+//   2000     0.304  0.317  0.200  0.192   0.227  0.259
+//   4096     0.589  0.611  0.383  0.373   0.429  0.494
+//   8192     1.181  1.203  1.076  0.987   0.887  1.199
+//   12000    1.824  1.766  2.169  1.715   1.916  2.133
+//   16384    2.737  2.825  3.445  3.026   2.913  3.332
+// The 64-row geometries leave CUs idle below m = 64 * 256 rows; the 16- and 32-row ones
+// fill them but cost more per row (per-workgroup coefficient traffic, and workgroups that
+// drift apart: at m = 16384 r16's workgroups take 1.9 to 3.4 ms).  Also measured at 16384:
+// wide with Q prefetched three intervals ahead 2.57 ms; two passes of 32 layers with two
+// workgroups per CU 3.11 ms (KG 8, 4 waves) and 3.55 ms (KG 4, 8 waves); neighbour-to-
+// neighbour interval counters in LDS instead of the workgroup barrier 2.66 ms.  Float32,
+// m = n, k = 64 (ms): 2000: wide 0.276, r16deep 0.164; 4096: 0.531, 0.309; 8192: 1.030,
+// 0.700; 16384: 2.038, 2.191 (r32 0.22 / 0.42 / 0.88 / 2.15).
+template <typename T>
+int choose_geom(const sep_handle* h, int64_t m, int64_t k)
+{
+    if (h && h->apply_geom) return h->apply_geom;  // SEP_APPLY (tests force each in turn)
+    if (k <= 32) return kGeomSmall;
+    if (sizeof(T) == 4) return m <= 12288 ? kGeomR16Deep : kGeomWide;
+    if (m <= 6144 || (m > 10240 && m <= 14336)) return kGeomR16Deep;
+    if (m <= 10240) return kGeomR32;
+    return kGeomWide;
+}
+
+template <typename T>
+size_t table_bytes(int geom, int64_t n)
+{
+    switch (geom) {
+        case kGeomWide: return sizeof(T) * DppDims<GeomWide>(n).table_elems();
+        case kGeomSmall: return sizeof(T) * DppDims<GeomSmall>(n).table_elems();
+        case kGeomR16Deep: return sizeof(T) * R16Dims<G16Deep>(n).table_elems();
+        default: return sizeof(T) * R16Dims<G32Wide>(n).table_elems();
+    }
+}
+
+template <typename T>
+int launch_geom(int geom, sep_handle* th, hipStream_t st, const ApplyMem<T>& mem, int64_t m,
+                int64_t n, T* A, int64_t lda, const T* cs, int64_t ld_cs, int64_t k,
+                bool rotations)
+{
+    switch (geom) {
+        case kGeomWide:
+            return run_dpp<T, GeomWide>(th, st, mem, m, n, A, lda, cs, ld_cs, k, rotations);
+        case kGeomSmall:
+            return run_dpp<T, GeomSmall>(th, st, mem, m, n, A, lda, cs, ld_cs, k, rotations);
+        case kGeomR16Deep:
+            return run_r16<T, G16Deep>(th, st, mem, m, n, A, lda, cs, ld_cs, k, rotations);
+        default:
+            return run_r16<T, G32Wide>(th, st, mem, m, n, A, lda, cs, ld_cs, k, rotations);
+    }
+}
+
+int check_apply_range(int64_t m, int64_t n)
+{
+    if (m > INT32_MAX || n > (INT32_MAX / 4)) {
+        set_error("bulk apply: m=%lld n=%lld beyond the 32-bit index range", (long long)m,
+                  (long long)n);
+        return SEP_ERR_ARG;
+    }
+    return SEP_OK;
+}
+
+constexpr size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
 }  // namespace
 
 template <typename T>
@@ -1059,50 +1178,79 @@ int bulk_apply_dev(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64
                    int64_t lda, const T* cs, int64_t ld_cs, int64_t k, bool rotations)
 {
     if (m <= 0 || n <= 1 || k <= 0) return SEP_OK;
-    if (m > INT32_MAX || n > (INT32_MAX / 4)) {
-        set_error("bulk apply: m=%lld n=%lld beyond the 32-bit index range", (long long)m,
-                  (long long)n);
-        return SEP_ERR_ARG;
-    }
-    // Measured kernel time per apply, m = n, k = 64 (ms):
-    //            wide   deep   r16    r16deep r32    r32deep
-    //   2000     0.304  0.317  0.200  0.192   0.227  0.259
-    //   4096     0.589  0.611  0.383  0.373   0.429  0.494
-    //   8192     1.181  1.203  1.076  0.987   0.887  1.199
-    //   12000    1.824  1.766  2.169  1.715   1.916  2.133
-    //   16384    2.737  2.825  3.445  3.026   2.913  3.332
-    // The 64-row geometries leave CUs idle below m = 64 * 256 rows; the 16- and 32-row ones
-    // fill them but cost more per row (per-workgroup coefficient traffic, and workgroups
-    // that drift apart: at m = 16384 r16's workgroups take 1.9 to 3.4 ms).  Also measured
-    // at 16384: wide with Q prefetched three intervals ahead 2.57 ms; two passes of 32
-    // layers with two workgroups per CU 3.11 ms (KG 8, 4 waves) and 3.55 ms (KG 4, 8
-    // waves); neighbour-to-neighbour interval counters in LDS instead of the workgroup
-    // barrier 2.66 ms.
-    // SEP_APPLY forces one of the four geometries the size rule below chooses between
-    // (tests run every shape through each of them).
-    const int geom = h ? h->apply_geom : 0;
-    if (geom == 2) return run_dpp<T, GeomWide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if (geom == 5) return run_dpp<T, GeomSmall>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if (geom == 7) return run_r16<T, G16Deep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if (geom == 9) return run_r16<T, G32Wide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if (k <= 32) return run_dpp<T, GeomSmall>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if constexpr (sizeof(T) == 4) {
-        // Float32, m = n, k = 64 (ms): 2000: wide 0.276, r16deep 0.164; 4096: 0.531, 0.309;
-        // 8192: 1.030, 0.700; 16384: 2.038, 2.191 (r32 0.22 / 0.42 / 0.88 / 2.15).  The
-        // scalar-fed kernel of apply.hip (SEP_APPLY=sgpr) takes 0.721 / 1.361 / 2.649 / 5.849.
-        if (m <= 12288) return run_r16<T, G16Deep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-        return run_dpp<T, GeomWide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    }
-    if (m <= 6144 || (m > 10240 && m <= 14336))
-        return run_r16<T, G16Deep>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    if (m <= 10240) return run_r16<T, G32Wide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
-    return run_dpp<T, GeomWide>(h, st, w, m, n, A, lda, cs, ld_cs, k, rotations);
+    if (int rc = check_apply_range(m, n)) return rc;
+    const int geom = choose_geom<T>(h, m, k);
+    if (int rc = w.table.reserve(table_bytes<T>(geom, n))) return rc;
+    if (int rc = w.aux.reserve(aux_bytes(n, sizeof(T)))) return rc;
+    return launch_geom<T>(geom, h, st, apply_mem<T>(w.table.p, w.aux.p, n), m, n, A, lda, cs,
+                          ld_cs, k, rotations);
 }
 
 template int bulk_apply_dev<double>(sep_handle*, hipStream_t, ApplyWork&, int64_t, int64_t,
                                     double*, int64_t, const double*, int64_t, int64_t, bool);
 template int bulk_apply_dev<float>(sep_handle*, hipStream_t, ApplyWork&, int64_t, int64_t,
                                    float*, int64_t, const float*, int64_t, int64_t, bool);
+
+// ---- the allocation-free apply --------------------------------------------------------
+// Scratch of sep_bulk_rmul_f64_dev_noalloc: its coefficient table and auxiliary words live
+// in this file-scope device array (one copy per GPU, held until the process ends) instead
+// of handle buffers.  In host code `scratch` only names the symbol: kernels get the
+// per-device address that hipGetSymbolAddress returns, recorded by scratch_init.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) {
+        set_error("sep_create: device %d beyond the %d devices of the scratch table", device,
+                  kMaxDevices);
+        return SEP_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return SEP_OK;
+    void* p = nullptr;
+    SEP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));  // the current device's copy
+    if (!p) {
+        set_error("hipGetSymbolAddress(scratch) returned a null address on device %d", device);
+        return SEP_ERR_HIP;
+    }
+    g_scratch_ptr[device] = p;
+    return SEP_OK;
+}
+
+int bulk_apply_noalloc_f64(sep_handle* h, int64_t m, int64_t n, double* A, int64_t lda,
+                           const double* cs, int64_t k)
+{
+    if (m <= 0 || n <= 1 || k <= 0) return SEP_OK;
+    if (int rc = check_apply_range(m, n)) return rc;
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        base = (h->device >= 0 && h->device < kMaxDevices) ? g_scratch_ptr[h->device] : nullptr;
+    }
+    if (!base) {
+        set_error("sep_bulk_rmul_f64_dev_noalloc: no scratch address for device %d", h->device);
+        return SEP_ERR_HIP;
+    }
+    const int geom = choose_geom<double>(h, m, k);
+    const size_t tb = round256(table_bytes<double>(geom, n));
+    const size_t need = tb + aux_bytes(n, sizeof(double));
+    if (need > kNoAllocScratchBytes) {
+        set_error("sep_bulk_rmul_f64_dev_noalloc: n=%lld needs %zu bytes of scratch, above the "
+                  "fixed %zu (use sep_bulk_rmul_f64_dev)",
+                  (long long)n, need, (size_t)kNoAllocScratchBytes);
+        return SEP_ERR_ARG;
+    }
+    unsigned char* b = static_cast<unsigned char*>(base);
+    // no timing events: the instrumentation may create them
+    return launch_geom<double>(geom, nullptr, h->stream, apply_mem<double>(b, b + tb, n), m, n,
+                               A, lda, cs, n - 1, k, false);
+}
 
 
 }  // namespace sep

@@ -238,6 +238,7 @@ int sep_create(sep_handle** out, int device)
                   prop.gcnArchName);
         return SEP_ERR_UNSUPPORTED;
     }
+    if (int rc = scratch_init(device)) return rc;
     sep_handle* h = new sep_handle();
     h->device = device;
     h->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
@@ -399,6 +400,26 @@ int sep_bulk_rmul_f64_dev(sep_handle* h, int64_t m, int64_t n, double* A, int64_
                           const double* cs, int64_t k)
 {
     return bulk_rmul_dev_checked<double>(h, m, n, A, lda, cs, k);
+}
+int sep_bulk_rmul_f64_dev_noalloc(sep_handle* h, int64_t m, int64_t n, double* A, int64_t lda,
+                                  const double* cs, int64_t k)
+{
+    if (!h) {
+        set_error("sep_bulk_rmul_f64_dev_noalloc: null handle");
+        return SEP_ERR_ARG;
+    }
+    if (m < 0 || n < 0 || k < 0 || lda < (m > 1 ? m : 1)) {
+        set_error("sep_bulk_rmul_f64_dev_noalloc: invalid sizes m=%lld n=%lld k=%lld lda=%lld",
+                  (long long)m, (long long)n, (long long)k, (long long)lda);
+        return SEP_ERR_ARG;
+    }
+    if (m == 0 || n <= 1 || k == 0) return SEP_OK;
+    if (!A || !cs) {
+        set_error("sep_bulk_rmul_f64_dev_noalloc: null pointer");
+        return SEP_ERR_ARG;
+    }
+    SEP_HIP(hipSetDevice(h->device));
+    return bulk_apply_noalloc_f64(h, m, n, A, lda, cs, k);
 }
 int sep_bulk_rmul_f32_dev(sep_handle* h, int64_t m, int64_t n, float* A, int64_t lda,
                           const float* cs, int64_t k)
