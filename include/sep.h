@@ -81,7 +81,7 @@ int sep_bulk_rmul_f32_dev(sep_handle *h, int64_t m, int64_t n, float *A, int64_t
  * for callers that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  Same arguments, kernels and results
  * (bit for bit) as sep_bulk_rmul_f64_dev; its coefficient table and auxiliary words live
- * in a fixed 40 MiB device array per GPU (held from the first sep_create on that GPU to
+ * in a fixed 48 MiB device array per GPU (held from the first sep_create on that GPU to
  * the end of the process), which bounds n: n <= 16384 always fits, a larger n that does
  * not fit returns SEP_ERR_ARG without launching anything.  It calls no hipMalloc/hipFree
  * and records no memset.  Every call on one GPU shares that array: calls that overlap on

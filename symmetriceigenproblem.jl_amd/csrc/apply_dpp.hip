@@ -37,20 +37,19 @@
 //  * Only the middle waves load coefficients (for every wave, one interval ahead, staged
 //    through LDS); the first wave issues only Q loads and the last only Q stores, so no
 //    wave ever has to wait for a memory operation it issued for a later interval.
-//  * Rotations with c^2 + s^2 = 1 (all rotations the QR chase produces, and the benchmark's
-//    cos/sin pairs) are applied as three shears, a += p b; b += q a; a += p b with
-//    p = s~/(1 + c~) = tan(phi/2), q = -s~: three in-place FMAs per rotation and row instead
-//    of two multiplies and two FMAs.  To keep |p| <= 1 the rotation is taken with c~ = |c|;
-//    the sign this drops is tracked per column (each rotation with c < 0 negates both of
-//    its columns in the stored representation) and folded into s~.  The systolic kernel
-//    multiplies each column by its final sign when it enters the pass (first wave), so it
-//    leaves with a positive stored sign; the 16- and 32-row kernels remove the sign when
-//    the column leaves.  The column signs are prefix products over the layers, computed
-//    by a small per-column kernel.
-//  * Grids that are not rotations (the reference benchmark's (rand, rand) pairs) take a
-//    five-instruction path, t = a; a += (c-1) a; a += s b; b += (c-1) b; b -= s t, chosen
-//    on the device by a normalisation check, so the entry point accepts any (c, s) as the
-//    reference's kernels do.
+//  * Rotations are applied as three shears, a += p b; b += q a; a += p b: three in-place
+//    FMAs per rotation and row instead of two multiplies and two FMAs.  For a rotation
+//    (c^2 + s^2 = 1) p = s~/(1 + |c|) = tan(phi/2), q = -s~, with the sign of c tracked
+//    per column; in general a real scale per column is tracked (k_colscale), which takes
+//    each rotation's sign(c) sqrt(c^2 + s^2), so pairs that are near normalised (all the
+//    QR chase produces, the benchmark's cos/sin pairs, and pairs within 1/64 of the unit
+//    circle) are applied as given, to rounding.  The systolic kernel multiplies each
+//    column by its final scale when it enters the pass (first wave), so it leaves with
+//    scale one; the 16- and 32-row kernels apply the scale when the column leaves.
+//  * Grids with pairs farther from the unit circle (the reference benchmark's
+//    (rand, rand)) take a five-instruction path, t = a; a += (c-1) a; a += s b;
+//    b += (c-1) b; b -= s t, chosen on the device by a check of the grid, so the entry
+//    point accepts any (c, s) as the reference's kernels do.
 #include <mutex>
 #include <type_traits>
 
@@ -61,24 +60,23 @@ namespace {
 
 constexpr int kLanes = 64;
 
-// flag = 1 if any rotation of the grid is not normalised: |c^2 + s^2 - 1| > 2 eps(T).
-// The residual is evaluated as fma(c, c, fma(s, s, -1)), accurate to a fraction of eps,
-// and cos/sin pairs rounded to T (the benchmark's, and any correctly rounded pair) stay
-// below 0.71 eps.  Inside the bound the three-shear form realises a rotation that differs
-// from the given (c, s) by at most the residual per entry; outside it the grid takes the
-// five-instruction path, which applies (c, s) as given.
+// flag = 1 if the grid needs the five-instruction path: some |c^2 + s^2 - 1| above 1/64,
+// or a c or s that is not finite.  Every other grid runs the scaled three-shear form (see
+// k_colscale), which applies the given (c, s) to rounding: normalised pairs, and pairs
+// that are only near normalised (c^2 + s^2 = 1 + delta), alike.  The residual is
+// evaluated as fma(c, c, fma(s, s, -1)), accurate to a fraction of eps.
 template <typename T>
 __global__ void __launch_bounds__(256) k_check_norm(const T* __restrict__ cs, int64_t ld,
                                                     int n, int k, int* __restrict__ flag)
 {
     const int64_t total = (int64_t)(n - 1) * k;
-    constexpr T kTol = sizeof(T) == 8 ? T(0x1p-51) : T(0x1p-22);  // 2 eps
+    constexpr T kBand = T(1) / T(64);
     int bad = 0;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
          q += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = q % (n - 1), L = q / (n - 1);
         const T c = cs[2 * (i + ld * L)], s = cs[2 * (i + ld * L) + 1];
-        bad |= !(fabs(fma(c, c, fma(s, s, T(-1)))) <= kTol);
+        bad |= !(fabs(fma(c, c, fma(s, s, T(-1)))) <= kBand);
     }
     if (__any(bad) && (threadIdx.x % kLanes) == 0) atomicOr(flag, 1);
 }
@@ -90,24 +88,30 @@ __global__ void __launch_bounds__(64) k_zero_flag(int* __restrict__ flag)
     if (threadIdx.x == 0) flag[0] = 0;
 }
 
+// Layers of one pass of every geometry (the scale table holds this many per column).
+constexpr int kMaxPassLayers = 64;
+
 // Device memory of one apply call: the coefficient table and the auxiliary words (column
-// sign masks, output signs, normalisation flag).
+// scales, input / output scale factors, normalisation flag).
 template <typename T>
 struct ApplyMem {
     T* table;
-    uint64_t* mask;
-    T* sigm1;
+    T* escale;  // [n][kMaxPassLayers]: column scale before each layer of the pass
+    T* sigm1;   // [n]: the column's scale factor at the pass boundary, minus one
     int* flag;
 };
 
 // The auxiliary words of an n-column apply, laid out from `aux` (256-byte aligned).
-inline size_t aux_bytes(int64_t n, size_t tsize) { return (8 + tsize) * (size_t)n + 256; }
+inline size_t aux_bytes(int64_t n, size_t tsize)
+{
+    return tsize * (size_t)n * (kMaxPassLayers + 1) + 256;
+}
 template <typename T>
 ApplyMem<T> apply_mem(void* table, void* aux, int64_t n)
 {
-    uint64_t* mask = static_cast<uint64_t*>(aux);
-    T* sigm1 = reinterpret_cast<T*>(mask + n);
-    return ApplyMem<T>{static_cast<T*>(table), mask, sigm1, reinterpret_cast<int*>(sigm1 + n)};
+    T* e = static_cast<T*>(aux);
+    T* sigm1 = e + (size_t)n * kMaxPassLayers;
+    return ApplyMem<T>{static_cast<T*>(table), e, sigm1, reinterpret_cast<int*>(sigm1 + n)};
 }
 
 // The reference's identity skip: bulk_wave_order_2x2_rmul! skips a fused block of four
@@ -132,74 +136,94 @@ __device__ __forceinline__ bool rot_skipped(const T* __restrict__ cs, int64_t ld
     return true;
 }
 
-// c < 0 of an applied rotation (a skipped or out-of-grid slot is the identity).
+// The scaled three-shear form.  Rotation (i, L) maps its columns a = col i, b = col i+1 to
+// a' = c a + s b, b' = c b - s a (Rotation2 rmul!).  With rho = sqrt(c^2 + s^2) and the
+// signed rho_s = sign(c) rho, it is rho_s times a rotation with C = |c| / rho >= 0,
+// S = s / rho_s.  Columns are stored as x^ = e x with a per-column scale e that absorbs
+// every rho_s: each rotation divides the scales of both its columns by rho_s, so on the
+// stored values it is  a^' = C a^ + S r b^,  b^' = -(S / r) a^ + C b^  with r = e_a / e_b,
+// a matrix of determinant C^2 + S^2 = 1, which three shears realise exactly:
+//   a^ += p b^;  b^ += q a^;  a^ += p b^   with  p = S r / (1 + C),  q = -S / r.
+// |p| <= |r| and |q| <= 1/|r|, and r stays near 1 inside the band k_check_norm admits.
+// A plain rotation has rho = 1 and e = +-1: that is sign tracking.  Grids whose pairs are
+// only near normalised thus cost no accuracy: the determinant 1 + delta of each rotation
+// goes into the scales instead of being dropped.  Skipped and out-of-grid slots have
+// rho_s = 1.
 template <typename T>
-__device__ __forceinline__ bool neg_c(const T* __restrict__ cs, int64_t ld, int n, int64_t i,
-                                      int L, int k)
+__device__ __forceinline__ T rho_signed(T c, T s)
 {
-    if (i < 0 || i > n - 2 || L >= k) return false;
-    if (!(cs[2 * (i + ld * L)] < T(0))) return false;
-    return !rot_skipped<T>(cs, ld, n, i, L, k);
+    const T rho = sqrt(fma(c, c, s * s));
+    return c < T(0) ? -rho : rho;
+}
+template <typename T>
+__device__ __forceinline__ T applied_rho(const T* __restrict__ cs, int64_t ld, int n, int64_t i,
+                                         int L, int k)
+{
+    if (i < 0 || i > n - 2 || L >= k || rot_skipped<T>(cs, ld, n, i, L, k)) return T(1);
+    return rho_signed<T>(cs[2 * (i + ld * L)], cs[2 * (i + ld * L) + 1]);
 }
 
-// Per column j: bit lr of mask[j] = the stored sign of column j is negative before the
-// rotations of layer L0 + lr; sigm1[j] = (final sign) - 1, i.e. 0 or -2.  With in_fix the
-// column is stored with its final sign from the start (the kernel multiplies it by that
-// sign when it enters the pass), so the bits are taken relative to it and the column
-// leaves the pass with a positive stored sign.
+// Column scales of one pass (layers L0 .. L0 + Kp - 1).  d_j(lr) is the product of the
+// rho_s of the rotations that touched column j before layer L0 + lr ((j-1, L) then (j, L)
+// in each layer), D_j = d_j(Kp).  The stored column is x^ = e x with e = D_j / d_j when
+// in_fix (the kernel multiplies the column by D_j as it enters the pass, and it leaves
+// with e = 1), and e = 1 / d_j otherwise (it enters unscaled, and the kernel multiplies it
+// by D_j as it leaves).  escale[j][lr] = e before layer L0 + lr, sigm1[j] = D_j - 1 (zero
+// on the five-instruction path, which tracks nothing).
 template <typename T>
-__global__ void __launch_bounds__(256) k_colsigns(const T* __restrict__ cs, int64_t ld,
-                                                  int n, int k, int L0, int Kp, int in_fix,
+__global__ void __launch_bounds__(256) k_colscale(const T* __restrict__ cs, int64_t ld, int n,
+                                                  int k, int L0, int Kp, int in_fix,
                                                   const int* __restrict__ flag,
-                                                  uint64_t* __restrict__ mask,
-                                                  T* __restrict__ sigm1)
+                                                  T* __restrict__ escale, T* __restrict__ sigm1)
 {
     // 64 columns per block, four threads per column, each over 16 of the (at most 64)
-    // layers: the chunk's sign bits as if the column entered it positive, and the chunk's
-    // parity; the chunks are then joined in layer order (a chunk entered negative has all
-    // its bits inverted)
-    __shared__ uint64_t cm[4][64];
-    __shared__ int cp[4][64];
+    // layers: the running product within its chunk, then the chunks' products in order
+    __shared__ T cp[4][64];
     const int c = threadIdx.x % 64, q = threadIdx.x / 64;
     const int j = blockIdx.x * 64 + c;
     const bool general = *flag != 0;
-    bool neg = false;
-    uint64_t m = 0;
-    if (j < n && !general) {
+    T part[16];  // product of the chunk's layers before layer q 16 + u
+    T run = T(1);
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int lr = q * 16 + u, L = L0 + lr;
-            if (lr < Kp && L < k) {
-                if (neg) m |= 1ull << lr;
-                // rotations (j-1, L) then (j, L) touch column j in layer L
-                neg ^= neg_c<T>(cs, ld, n, j - 1, L, k) ^ neg_c<T>(cs, ld, n, j, L, k);
-            }
-        }
+    for (int u = 0; u < 16; ++u) {
+        const int lr = q * 16 + u, L = L0 + lr;
+        part[u] = run;
+        if (j < n && !general && lr < Kp && L < k)
+            run *= applied_rho<T>(cs, ld, n, j - 1, L, k) * applied_rho<T>(cs, ld, n, j, L, k);
     }
-    cm[q][c] = m;
-    cp[q][c] = neg;
+    cp[q][c] = run;
     __syncthreads();
-    if (q != 0 || j >= n) return;
-    if (general) {
-        mask[j] = 0;
-        sigm1[j] = 0.0;
-        return;
-    }
-    uint64_t mm = 0;
-    bool pre = false;
+    if (j >= n) return;
+    T pre = T(1), D = T(1);
     for (int t = 0; t < 4; ++t) {
-        const uint64_t valid = ((t * 16 + 16 <= Kp) ? 0xFFFFull
-                                                    : ((1ull << (Kp > t * 16 ? Kp - t * 16 : 0)) - 1))
-                               << (16 * t);
-        const uint64_t lim = (L0 + t * 16 < k)
-                                 ? (((k - L0) >= 64) ? ~0ull : ((1ull << (k - L0)) - 1))
-                                 : 0ull;
-        mm |= pre ? ((cm[t][c] ^ valid) & lim & valid) : cm[t][c];
-        pre ^= cp[t][c] != 0;
+        if (t < q) pre *= cp[t][c];
+        D *= cp[t][c];
     }
-    if (in_fix && pre) mm = ~mm;
-    mask[j] = mm;
-    sigm1[j] = pre ? -2.0 : 0.0;
+    if (general) D = T(1);
+    const T num = in_fix ? D : T(1);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int lr = q * 16 + u;
+        if (lr < kMaxPassLayers) escale[(int64_t)j * kMaxPassLayers + lr] = num / (pre * part[u]);
+    }
+    if (q == 0) sigm1[j] = D - T(1);
+}
+
+// Shear coefficients (p, q) of rotation (i, L) = layer L0 + lr, applied, from the scales.
+template <typename T>
+__device__ __forceinline__ void shear_coefs(const T* __restrict__ cs, int64_t ld, int n, int64_t i,
+                                            int L, int k, int lr, const T* __restrict__ escale,
+                                            T& p, T& qq)
+{
+    const T c = cs[2 * (i + ld * L)], s = cs[2 * (i + ld * L) + 1];
+    const T rs = rho_signed<T>(c, s);
+    const T C = fabs(c) / fabs(rs), S = s / rs;
+    // scale of column i after rotation (i-1, L) touched it, of column i+1 before layer L
+    const T ea = escale[i * kMaxPassLayers + lr] / applied_rho<T>(cs, ld, n, i - 1, L, k);
+    const T eb = escale[(i + 1) * kMaxPassLayers + lr];
+    const T r = ea / eb;
+    p = S * r / (T(1) + C);
+    qq = -S / r;
 }
 
 // Coefficient table of one pass: tab[g][u][2 KG] for g < G, u = t + S + PADU (time t,
@@ -207,7 +231,7 @@ __global__ void __launch_bounds__(256) k_colsigns(const T* __restrict__ cs, int6
 // inside the table).  Slot l of wave g at time t is rotation i = t - 2 (g KG + l) of layer
 // L0 + g KG + l; [2l] = p, [2l+1] = q (shear form) or c - 1, s (general form);
 // skipped (rot_skipped) and out-of-grid slots hold (0, 0), the identity in both forms.
-// Section g = G holds, for every interval bt, the first wave's 16-lane input-sign row
+// Section g = G holds, for every interval bt, the first wave's 16-lane input-scale row
 // (sigm1 of the columns bt S + 1 .. bt S + S that enter the pass in that interval) at the
 // position a coefficient row of a (virtual) wave G would have, so the coefficient loaders
 // fetch it with the same address pattern.
@@ -216,7 +240,7 @@ __global__ void __launch_bounds__(256) k_coef_table(T* __restrict__ tab,
                                                     const T* __restrict__ cs, int64_t ld,
                                                     int n, int k, int L0, int Up, int PADU,
                                                     int NI, const int* __restrict__ flag,
-                                                    const uint64_t* __restrict__ mask,
+                                                    const T* __restrict__ escale,
                                                     const T* __restrict__ sigm1)
 {
     constexpr int NC = 2 * KG;
@@ -225,7 +249,7 @@ __global__ void __launch_bounds__(256) k_coef_table(T* __restrict__ tab,
     const int general = *flag;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
          q += (int64_t)gridDim.x * blockDim.x) {
-        if (q >= rot_slots) {  // output-sign section
+        if (q >= rot_slots) {  // input-scale section
             const int qq = (int)(q - rot_slots);
             const int bt = qq / 16, c = qq % 16;
             const int col = bt * S + 1 + c;
@@ -247,13 +271,7 @@ __global__ void __launch_bounds__(256) k_coef_table(T* __restrict__ tab,
                 p = c - 1.0;
                 qq = s;
             } else {
-                bool na = (mask[i] >> lr) & 1;
-                na ^= neg_c<T>(cs, ld, n, i - 1, L, k);
-                const bool nb = (mask[i + 1] >> lr) & 1;
-                const T ct = fabs(c);
-                const T st = ((c < 0.0) ^ na ^ nb) ? -s : s;
-                p = st / (1.0 + ct);
-                qq = -st;
+                shear_coefs<T>(cs, ld, n, i, L, k, lr, escale, p, qq);
             }
         }
         tab[2 * q] = p;
@@ -385,8 +403,8 @@ __device__ __forceinline__ void step_general(T (&w)[WR], T cf)
 #undef SLOT
 }
 
-// Output sign fix of the last wave: x += (sigma - 1) x, with (sigma - 1) in {0, -2}
-// broadcast from lane s of the interval's sign row (exact: the result is x or -x).
+// Scale of a column entering the pass (first wave): x += (D - 1) x, with D - 1 broadcast
+// from lane s of the interval's scale row (exact when D = +-1, one rounding otherwise).
 #define SEP_FIX(N) asm("s_nop 1\n" SEP_DPP("%0", "%1", "%0", N) : "+v"(x) : "v"(sg))
 
 template <typename T, int s>
@@ -525,7 +543,7 @@ __device__ __forceinline__ void interval(T* At, uint32_t voff, int64_t lda, int 
         if constexpr (s % (S / 4) == 0) __builtin_amdgcn_s_setprio(3 - s / (S / 4));
         read_in(std::integral_constant<int, s + LA>{});
         if constexpr (FIRST) {
-            // the column enters the pass with its final sign (see k_colsigns)
+            // the column enters the pass with its final scale (see k_colscale)
             win[(s + 1) % W] = pf[D][s];
             if constexpr (!GENERAL) fix_sign<T, s>(win[(s + 1) % W], sg);
             pf[D][s] = col_load<T>(pq, c0f + s, n, voff);
@@ -549,7 +567,7 @@ __device__ __forceinline__ void interval(T* At, uint32_t voff, int64_t lda, int 
 }
 
 // LDS layout of one interval's coefficient block: rows [g][r][16] for every wave, then
-// the last wave's 16-lane output-sign row.
+// the first wave's 16-lane input-scale row.
 template <class Gm>
 struct CoefBlock {
     static constexpr int ROWS = Gm::G * Gm::NR * 16;
@@ -583,7 +601,7 @@ __device__ __forceinline__ void block_offsets(int (&off)[CoefBlock<Gm>::PER], in
 }
 
 // The middle waves fetch the next interval's coefficient block (every wave's rows and
-// the output signs) from the table into registers at the start of an interval and store
+// the input scales) from the table into registers at the start of an interval and store
 // it to LDS at the end; they issue no other global memory operations, so waiting for
 // these loads never drains the first wave's Q prefetch or the last wave's stores.
 template <typename T, class Gm>
@@ -633,7 +651,7 @@ __device__ __forceinline__ void wave(T* __restrict__ A, int64_t lda, int m, int 
     if constexpr (FIRST) {
         // column 0 enters before step 0 (slot 0); interval d takes columns dS+1 .. dS+S
         win[0] = col_load<T>(At, 0, n, voff);
-        if constexpr (!GENERAL) win[0] = fma(sigm1[0], win[0], win[0]);  // its final sign
+        if constexpr (!GENERAL) win[0] = fma(sigm1[0], win[0], win[0]);  // its final scale
 #pragma unroll
         for (int d = 0; d < PFD; ++d)
 #pragma unroll
@@ -676,7 +694,7 @@ __device__ __forceinline__ void wave(T* __restrict__ A, int64_t lda, int m, int 
             const int t0 = (b - g) * S + g;
             const T* cb = cbuf[b & 1];
             const T* cbw = cb + g * NR * 16 + cl;  // this wave's coefficient rows
-            const T sg = FIRST ? cb[CB::ROWS + cl] : 0.0;  // input signs
+            const T sg = FIRST ? cb[CB::ROWS + cl] : 0.0;  // input scales
             const T* lds_in = FIRST ? nullptr : &ring[g > 0 ? g - 1 : 0][(b - 1) & 1][0][lane];
             T* lds_out = LAST ? nullptr : &ring[g][b & 1][0][lane];
             interval<T, Gm, FIRST, LAST, GENERAL, D>(At, voff, lda, n, win, pf, cbw, sg, lds_in,
@@ -743,7 +761,7 @@ int run_dpp(sep_handle* th, hipStream_t st, const ApplyMem<T>& mem, int64_t m, i
     sep_handle* h = th;
     const DppDims<Gm> dd(n);
     const int NI = dd.NI, PADU = dd.PADU, Up = dd.Up;
-    uint64_t* mask = mem.mask;
+    T* escale = mem.escale;
     T* sigm1 = mem.sigm1;
     int* flag = mem.flag;
     const int tp0 = timing_mark(h, st);
@@ -759,14 +777,14 @@ int run_dpp(sep_handle* th, hipStream_t st, const ApplyMem<T>& mem, int64_t m, i
     const int64_t rb = (m + kLanes - 1) / kLanes;
     int tp_prev = tp0;
     for (int64_t L0 = 0; L0 < k; L0 += Kp) {
-        hipLaunchKernelGGL(k_colsigns<T>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st, cs,
-                           ld_cs, (int)n, (int)k, (int)L0, Kp, 1, flag, mask, sigm1);
+        hipLaunchKernelGGL(k_colscale<T>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st, cs,
+                           ld_cs, (int)n, (int)k, (int)L0, Kp, 1, flag, escale, sigm1);
         SEP_CHECK_LAUNCH();
         const int64_t total = (int64_t)G * Up * KG + (int64_t)NI * 16;
         const int tblocks = (int)std::min<int64_t>((total + 255) / 256, 8192);
         hipLaunchKernelGGL((k_coef_table<T, KG, G, S>), dim3(tblocks), dim3(256), 0, st,
                            mem.table, cs, ld_cs, (int)n, (int)k, (int)L0, Up, PADU, NI, flag,
-                           mask, sigm1);
+                           escale, sigm1);
         SEP_CHECK_LAUNCH();
         const int ta = timing_mark(h, st);
         timing_pair(h, false, tp_prev, ta);
@@ -796,7 +814,7 @@ using GeomSmall = Geom<4, 8, 8, 3>;   // 32 layers per pass (short grids)
 // hands out column u - DL.  Every wave runs the same FMA stream with no bounds tests
 // (columns outside the matrix are zeros under identity rotations); wave 0 additionally
 // loads A four columns per instruction (lane 16 c + j: column c, row j) and the last wave
-// stores, sign-fixes and reads back its outputs the same way.  With 16 rows per
+// stores, scales and reads back its outputs the same way.  With 16 rows per
 // workgroup, four independent workgroups share a CU, so the waves sharing a SIMD belong
 // to different workgroups and do not stall at the same barriers.
 constexpr int gcd_(int a, int b) { return b == 0 ? a : gcd_(b, a % b); }
@@ -842,7 +860,7 @@ __global__ void __launch_bounds__(256) k_coef_r16(T* __restrict__ tab,
                                                   const T* __restrict__ cs, int64_t ld,
                                                   int n, int k, int L0, int NBt,
                                                   const int* __restrict__ flag,
-                                                  const uint64_t* __restrict__ mask)
+                                                  const T* __restrict__ escale)
 {
     constexpr int S = Gm::S, KG = Gm::KG, NC = Gm::NC, SPR = Gm::SPR, NR = Gm::NR;
     const int64_t total = (int64_t)Gm::NW * NBt * NR * kLanes;
@@ -867,12 +885,9 @@ __global__ void __launch_bounds__(256) k_coef_r16(T* __restrict__ tab,
             if (general) {
                 v = (j & 1) ? s : c - 1.0;
             } else {
-                // stored signs of columns i and i+1 when rotation (i, L) is applied
-                bool na = (mask[i] >> lr) & 1;
-                na ^= neg_c<T>(cs, ld, n, i - 1, L, k);
-                const bool nb = (mask[i + 1] >> lr) & 1;
-                const T st = ((c < 0.0) ^ na ^ nb) ? -s : s;
-                v = (j & 1) ? -st : st / (1.0 + fabs(c));
+                T p, qq;
+                shear_coefs<T>(cs, ld, n, i, L, k, lr, escale, p, qq);
+                v = (j & 1) ? qq : p;
             }
         }
         tab[e] = v;
@@ -1065,7 +1080,7 @@ int run_r16(sep_handle* th, hipStream_t st, const ApplyMem<T>& mem, int64_t m, i
     const R16Dims<Gm> rd(n);
     const int NB = rd.NB, NBt = rd.NBt;
     const size_t tab_elems = rd.table_elems();
-    uint64_t* mask = mem.mask;
+    T* escale = mem.escale;
     T* sigm1 = mem.sigm1;
     int* flag = mem.flag;
     const int tp0 = timing_mark(h, st);
@@ -1081,12 +1096,12 @@ int run_r16(sep_handle* th, hipStream_t st, const ApplyMem<T>& mem, int64_t m, i
     const int64_t wgs = (m + Gm::RPG - 1) / Gm::RPG;
     int tp_prev = tp0;
     for (int64_t L0 = 0; L0 < k; L0 += Kp) {
-        hipLaunchKernelGGL(k_colsigns<T>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st, cs,
-                           ld_cs, (int)n, (int)k, (int)L0, Kp, 0, flag, mask, sigm1);
+        hipLaunchKernelGGL(k_colscale<T>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st, cs,
+                           ld_cs, (int)n, (int)k, (int)L0, Kp, 0, flag, escale, sigm1);
         SEP_CHECK_LAUNCH();
         const int tblocks = (int)std::min<int64_t>(((int64_t)tab_elems + 255) / 256, 8192);
         hipLaunchKernelGGL((k_coef_r16<T, Gm>), dim3(tblocks), dim3(256), 0, st, mem.table, cs,
-                           ld_cs, (int)n, (int)k, (int)L0, NBt, flag, mask);
+                           ld_cs, (int)n, (int)k, (int)L0, NBt, flag, escale);
         SEP_CHECK_LAUNCH();
         const int ta = timing_mark(h, st);
         timing_pair(h, false, tp_prev, ta);

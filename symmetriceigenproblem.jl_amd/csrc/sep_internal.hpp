@@ -140,10 +140,11 @@ template <typename T>
 int bulk_apply_dev(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64_t n, T* A,
                    int64_t lda, const T* cs, int64_t ld_cs, int64_t k, bool rotations = false);
 // The allocation-free Float64 apply: the same kernels with their table and auxiliary words in
-// a fixed device array of kNoAllocScratchBytes per GPU (40 MiB: n <= 16384 for every
-// geometry; the largest table, the 32-row geometry's, needs 34.4 MB there).  scratch_init
-// records the array's address on `device` (the current device).
-constexpr size_t kNoAllocScratchBytes = size_t(40) << 20;
+// a fixed device array of kNoAllocScratchBytes per GPU (48 MiB: n <= 16384 for every
+// geometry; the largest need, the 32-row geometry's table and the column scales, is
+// 42.7 MB there).  scratch_init records the array's address on `device` (the current
+// device).
+constexpr size_t kNoAllocScratchBytes = size_t(48) << 20;
 int scratch_init(int device);
 int bulk_apply_noalloc_f64(sep_handle* h, int64_t m, int64_t n, double* A, int64_t lda,
                            const double* cs, int64_t k);

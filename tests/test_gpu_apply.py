@@ -195,24 +195,45 @@ def test_s_zero_entries_follow_the_reference_block_skip(sep, monkeypatch, geom, 
         h.close()
 
 
-@pytest.mark.parametrize("dtype,delta", [(np.float64, 9e-15), (np.float64, -9e-15),
-                                         (np.float32, 5e-7), (np.float32, -5e-7)])
-def test_near_normalised_grid_outside_the_bound(sep, handle, dtype, delta):
-    """Pairs with c^2 + s^2 = 1 + delta, delta of one sign for every rotation and beyond the
-    normalisation bound (2 eps): the three-shear form would realise a determinant-one
-    rotation and drift by ~k delta, so these must take the path that applies (c, s) as
-    given, and match the oracle to the usual tolerance."""
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("delta_eps", [1.5, -1.5, 40.0, -40.0, 1e4, -1e4])
+def test_near_normalised_grids(sep, handle, dtype, delta_eps):
+    """Pairs with c^2 + s^2 = 1 + delta, delta of one sign for every rotation: just inside
+    and well beyond the residual of correctly rounded cos/sin pairs (0.71 eps), up to 1e4
+    eps.  A determinant-one (plain shear) realisation would drift by about k delta; the
+    kernel folds each pair's sign(c) sqrt(c^2 + s^2) into per-column scales, so every such
+    grid must match the oracle to the usual tolerance."""
     m, n, k = 64, 400, 64
+    eps = np.finfo(dtype).eps
     r = np.random.default_rng(17)
     th = r.random((k, n - 1)) * 2 * np.pi
-    f = np.sqrt(1.0 + delta)
+    # pick the scale in float64 so that the rounded pairs keep a residual of one sign
+    f = np.sqrt(1.0 + 2 * delta_eps * eps)
     g = np.ascontiguousarray(np.stack([np.cos(th) * f, np.sin(th) * f], -1).astype(dtype))
-    c, s = g[..., 0].astype(np.float64), g[..., 1].astype(np.float64)
-    res = np.abs(c * c + s * s - 1)
-    assert res.min() > 2 * np.finfo(dtype).eps  # every pair is outside the bound
+    # residual in extended precision (its products are exact to 2^-64)
+    c, s = g[..., 0].astype(np.longdouble), g[..., 1].astype(np.longdouble)
+    res = c * c + s * s - 1
+    assert np.all(np.sign(res) == np.sign(delta_eps)) and np.abs(res).max() < 1 / 64
     A = sep.inputs.gaussian_colmajor(m, n, 3, dtype)
     ref = A.copy(order="F")
     oracle.bulk_rmul_seq(ref, g)
     out = A.copy(order="F")
     sep.bulk_wave_order_2x2_rmul_(out, g, handle=handle)
     assert _relerr(out, ref) <= TOL[dtype], _relerr(out, ref)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_grid_beyond_the_band_takes_the_exact_path(sep, handle, dtype):
+    """c^2 + s^2 = 1.05 for every pair (beyond the 1/64 band of the scaled shear form):
+    the five-instruction path applies the pairs as given."""
+    m, n, k = 64, 400, 64
+    r = np.random.default_rng(19)
+    th = r.random((k, n - 1)) * 2 * np.pi
+    f = np.sqrt(1.05)
+    g = np.ascontiguousarray(np.stack([np.cos(th) * f, np.sin(th) * f], -1).astype(dtype))
+    A = sep.inputs.gaussian_colmajor(m, n, 5, dtype)
+    ref = A.copy(order="F")
+    oracle.bulk_rmul_seq(ref, g)
+    out = A.copy(order="F")
+    sep.bulk_wave_order_2x2_rmul_(out, g, handle=handle)
+    assert _relerr(out, ref) <= 10 * TOL[dtype], _relerr(out, ref)

@@ -58,7 +58,7 @@ def test_noalloc_rejects_orders_beyond_its_scratch(sep):
     import torch
     h = sep.Handle(0)
     try:
-        m, n, k = 8000, 40000, 64  # the 32-row geometry's table exceeds 40 MiB here
+        m, n, k = 8000, 40000, 64  # the 32-row geometry needs well over 48 MiB here
         A = torch.full((n, m), 3.0, dtype=torch.float64, device="cuda").t()
         g = torch.zeros((k, n - 1, 2), dtype=torch.float64, device="cuda")
         rc = _call(sep, "sep_bulk_rmul_f64_dev_noalloc", h, A, g)
