@@ -126,8 +126,8 @@ int sep_qr_algorithm_f32_dev(sep_handle *h, int64_t n, float *dv, float *ev, flo
  * The rounds are captured into a hipGraph once per buffer set, layout and configuration
  * and replayed (the environment variable SEP_BATCHED_GRAPH=0 at sep_create launches them
  * one by one instead).  Q = NULL computes eigenvalues only (no application).
- * The results equal per-problem sep_qr_algorithm_f64 up to the apply's rounding form
- * (here Rotation2 rmul! arithmetic, bit for bit the layer-major product).
+ * Eigenvalues, off-diagonal and iteration counts equal per-problem sep_qr_algorithm_f64's
+ * bit for bit (the same chase); Q agrees to rounding (the same three-shear apply form).
  * max_layers = 0 takes the autotuned (or default, 16) block length.  converged[p]
  * (int32) and iters[p] (int64) are written per problem.  Synchronous: returns after the
  * solve.  Returns SEP_NOT_CONVERGED if any problem did not converge. */

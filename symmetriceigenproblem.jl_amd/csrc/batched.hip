@@ -15,10 +15,11 @@
 //                 8 steps ahead, and the (c, s) of 16 time steps staged in LDS (double
 //                 buffered, loaded one stage ahead).  Rotations beyond a column's layer
 //                 count read as the identity, so the grid never needs an identity fill.
-//                 The rotation is a1' = fma(a1, c, a2 s), a2' = fma(a1, -s, a2 c) (Rotation2
-//                 rmul!, src/simple_givens.jl:53-64, contracted as the reference's SIMD
-//                 kernels may be, src/bulk_givens.jl:44-57), so Q equals the C
-//                 restatement's layer-major product with the same rotation bit for bit.
+//                 Each rotation (Rotation2 rmul!, src/simple_givens.jl:53-64) is applied
+//                 as three shears with DPP-broadcast coefficients, as in the bulk apply;
+//                 Q agrees with the layer-major product to rounding (eigenvalues, the
+//                 off-diagonal and the iteration counts come from the chase alone and
+//                 equal the single solve's bit for bit).
 // The grid, layer counts and depths are double buffered by round parity, so the apply of
 // round r (apply stream) overlaps the chase of round r+1 (chase stream).  The host reads a
 // device count of finished problems after each chase; every round either finishes a
@@ -32,6 +33,7 @@
 
 #include "chase.hpp"
 #include "sep_internal.hpp"
+#include "shear.hpp"
 
 namespace sep {
 namespace {
@@ -175,6 +177,13 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off
 // addressed through a buffer resource spanning exactly those columns, so the wave-order
 // loads of columns past the range read 0 and the stores of columns outside it (and of
 // lanes past the last row) are dropped without a branch.
+//
+// Each rotation is three in-place FMAs, a += p b; b += q a; a += p b with p = s / (1 + c),
+// q = -s (the shear form of the bulk apply, shear.hpp), fed by DPP broadcasts from one
+// coefficient register per 8 layers and step.  The chase's rotations come from DLARTG,
+// whose sign rule leaves c > 0 whenever |f| > |g| and |c| <= 1/sqrt 2 otherwise, so
+// 1 + c >= 1 - 1/sqrt 2 and |p| <= 1 + sqrt 2: no column signs need tracking.  Slots past
+// a column's layer count, outside the block or past its depth hold p = q = 0.
 template <int KF, int R>
 __device__ __forceinline__ void apply_item(const BWork wk, const BProb* __restrict__ probs,
                                            const int* __restrict__ lt,
@@ -186,8 +195,10 @@ __device__ __forceinline__ void apply_item(const BWork wk, const BProb* __restri
     constexpr int SB = W > kStageSteps ? W : kStageSteps;
     constexpr int PA = kQAhead;
     constexpr int NE = SB * KF / kLanes;  // staged entries per lane
+    constexpr int NCR = KF >= 8 ? KF / 8 : 1;  // coefficient registers per step
     static_assert(SB % W == 0 && SB % PA == 0 && NE * kLanes == SB * KF, "stage geometry");
-    __shared__ double2 stg[2][SB][KF];
+    static_assert(KF == 4 || KF % 8 == 0, "KF is 4 or a multiple of 8");
+    __shared__ double stc[2][SB][NCR][16];
 
     const int4 inf = info[wk.prob];
     const int depth = __builtin_amdgcn_readfirstlane(inf.x);
@@ -231,7 +242,11 @@ __device__ __forceinline__ void apply_item(const BWork wk, const BProb* __restri
         for (int e = 0; e < NE; ++e) {
             const int q = lane + kLanes * e;
             const int s = q / KF, l = q % KF;
-            stg[buf][s][l] = (sin_[e] && L0 + l < slt[e]) ? sv[e] : make_double2(1.0, 0.0);
+            const bool on = sin_[e] && L0 + l < slt[e];
+            const double p = on ? sv[e].y / (1.0 + sv[e].x) : 0.0;
+            const double qq = on ? -sv[e].y : 0.0;
+            stc[buf][s][l / 8][2 * (l % 8)] = p;
+            stc[buf][s][l / 8][2 * (l % 8) + 1] = qq;
         }
     };
 
@@ -269,18 +284,21 @@ __device__ __forceinline__ void apply_item(const BWork wk, const BProb* __restri
                     win[r][(s + 1) % W] = qn[r][k];  // column t + 1 enters the window
                     qn[r][k] = buf_load(qr, rowb[r] + ld0 + (uint32_t)s * colb);
                 }
-                Unroll<KF, 0>::run([&](auto lc) {
-                    constexpr int l = decltype(lc)::value;
-                    constexpr int ia = ((s - 2 * l) % W + W) % W;
-                    constexpr int ib = (ia + 1) % W;
-                    const double2 cs = stg[buf][s][l];
+                double cf[NCR];
 #pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        const double x = win[r][ia], y = win[r][ib];
-                        win[r][ia] = __builtin_fma(x, cs.x, y * cs.y);
-                        win[r][ib] = __builtin_fma(x, -cs.y, y * cs.x);
+                for (int h = 0; h < NCR; ++h) cf[h] = stc[buf][s][h][lane & 15];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if constexpr (KF == 4) {
+                        step_shear<double, 4, s, 0, W>(win[r], cf[0]);
+                    } else {
+                        Unroll<NCR, 0>::run([&](auto hc) {
+                            constexpr int h = decltype(hc)::value;
+                            // layers 8h .. 8h + 7: slots s - 2 (8h + l)
+                            step_shear<double, 8, s - 16 * h, 0, W>(win[r], cf[h]);
+                        });
                     }
-                });
+                }
 #pragma unroll
                 for (int r = 0; r < R; ++r)  // column t - 2 KF + 2 leaves the window
                     buf_store(qr, rowb[r] + st0 + (uint32_t)s * colb, win[r][(s + 2) % W]);

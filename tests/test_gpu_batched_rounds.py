@@ -1,13 +1,12 @@
 """GPU parity of the round-based batched solver (no counterpart in the reference).
 
 Every problem of a batch runs qr_algorithm! semantics (src/bulge_chasing.jl:146-288) with
-the single solve's device chase, and its block of rotations is applied in wave order with
-the rotation as two fused multiply-adds (Rotation2 rmul!, src/simple_givens.jl:53-64, as
-the reference's SIMD kernels contract it, src/bulk_givens.jl:44-57); wave order equals the
-layer-major product bit for bit.  So each problem's eigenvalues, off-diagonal,
-eigenvectors, iteration count and convergence flag are compared bit for bit with the
-oracle's qr_algorithm with the same rotation (apply_mode=3); full-size samples are checked
-against LAPACK and by residual / orthogonality.
+the single solve's device chase, and its block of rotations is applied in wave order as
+three shears per rotation (the bulk apply's form).  So each problem's eigenvalues,
+off-diagonal, iteration count and convergence flag are compared bit for bit with the
+oracle's qr_algorithm (they come from the chase alone), and its eigenvectors to
+||Q - Q_oracle||_F <= 1e-13 sqrt(n) (Q has Frobenius norm sqrt(n)); full-size batches are
+checked against LAPACK and by residual / orthogonality.
 """
 import numpy as np
 import pytest
@@ -38,7 +37,7 @@ def _solve(sep, handle, ns, d, e, off_d, off_q, max_layers=0, tune=False):
     return conv, iters, dvd.cpu().numpy(), evd.cpu().numpy(), Q.cpu().numpy()
 
 
-def _check_bitwise(ns, d, e, off_d, off_q, ml, conv, iters, gd, ge, gQ, which=None):
+def _check_oracle(ns, d, e, off_d, off_q, ml, conv, iters, gd, ge, gQ, which=None):
     for p in (range(len(ns)) if which is None else which):
         n, o = int(ns[p]), off_d[p]
         dv, ev = d[o:o + n].copy(), e[o:o + n - 1].copy()
@@ -48,7 +47,7 @@ def _check_bitwise(ns, d, e, off_d, off_q, ml, conv, iters, gd, ge, gQ, which=No
         assert np.array_equal(gd[o:o + n], dv), (p, n)
         assert np.array_equal(ge[o:o + n - 1], ev), (p, n)
         Qg = gQ[off_q[p]:off_q[p] + n * n].reshape(n, n).T   # column-major
-        assert np.array_equal(Qg, Qo), (p, n)
+        assert np.linalg.norm(Qg - Qo) <= 1e-13 * np.sqrt(n), (p, n, np.linalg.norm(Qg - Qo))
 
 
 @pytest.mark.parametrize("max_layers,fused_depth,rows_per_lane",
@@ -67,7 +66,7 @@ def test_rounds_bitwise_equal_to_oracle(sep, handle, max_layers, fused_depth, ro
     finally:
         handle.set_batched_config(0, 0)
     ml = max_layers if max_layers > 0 else 16  # the default block length
-    _check_bitwise(ns, d, e, off_d, off_q, ml, conv, iters, gd, ge, gQ)
+    _check_oracle(ns, d, e, off_d, off_q, ml, conv, iters, gd, ge, gQ)
 
 
 def test_rounds_full_size_sample_with_tuner(sep, handle):
@@ -92,7 +91,7 @@ def test_rounds_full_size_sample_with_tuner(sep, handle):
     # the tuned configuration changes only speed: an explicit block length gives the
     # oracle's bits whatever fused depth / rows per lane the tuner chose
     conv, iters, gd, ge, gQ = _solve(sep, handle, ns, d, e, off_d, off_q, 32)
-    _check_bitwise(ns, d, e, off_d, off_q, 32, conv, iters, gd, ge, gQ, which=range(0, 512, 43))
+    _check_oracle(ns, d, e, off_d, off_q, 32, conv, iters, gd, ge, gQ, which=range(0, 512, 43))
 
 
 def test_rounds_repeat_is_deterministic(sep, handle):
@@ -157,7 +156,7 @@ def test_graph_replays_match_eager_rounds(sep, monkeypatch):
             out = (conv, iters, dvd.cpu().numpy(), evd.cpu().numpy(), Q.cpu().numpy())
             for x, y in zip(out, ref):
                 assert np.array_equal(x, y), rep
-        _check_bitwise(ns, d, e, off_d, off_q, 16, *ref, which=range(0, len(ns), 9))
+        _check_oracle(ns, d, e, off_d, off_q, 16, *ref, which=range(0, len(ns), 9))
     finally:
         he.close()
         hg.close()
@@ -203,7 +202,7 @@ def test_odd_orders_extension(sep):
             assert bool(conv[p]) == oconv and iters[p] == oit, (p, n)
             assert np.array_equal(gd[o:o + n], dv) and np.array_equal(ge[o:o + n - 1], ev)
             Qg = gQ[off_q[p]:off_q[p] + n * n].reshape(n, n).T
-            assert np.array_equal(Qg, Qo), (p, n)
+            assert np.linalg.norm(Qg - Qo) <= 1e-13 * np.sqrt(n), (p, n)
     finally:
         h.close()
 
@@ -214,7 +213,7 @@ def test_rounds_large_orders(sep, handle, ns):
     order exceeds that chases from global memory: both bit for bit against the oracle."""
     ns, d, e, off_d, off_q = _batch(sep, ns, seed=int(sum(ns)))
     conv, iters, gd, ge, gQ = _solve(sep, handle, ns, d, e, off_d, off_q, max_layers=32)
-    _check_bitwise(ns, d, e, off_d, off_q, 32, conv, iters, gd, ge, gQ)
+    _check_oracle(ns, d, e, off_d, off_q, 32, conv, iters, gd, ge, gQ)
 
 
 def _tridiag_residuals(d, e, Q, lam):
@@ -252,4 +251,4 @@ def test_full_batch_4096_problems(sep, handle):
     conv, iters, gd, ge, gQ = _solve(sep, handle, ns, d, e, off_d, off_q, 16)
     which = sorted({int(np.argmax(ns))} | set(range(0, len(ns), 133)))
     assert len(which) >= 32
-    _check_bitwise(ns, d, e, off_d, off_q, 16, conv, iters, gd, ge, gQ, which=which)
+    _check_oracle(ns, d, e, off_d, off_q, 16, conv, iters, gd, ge, gQ, which=which)
