@@ -112,12 +112,16 @@ def time_apply(sep, h, torch, n, k, dtype, steps=5):
     return 6.0 * n * (n - 1) * k / (ms * 1e-3) / 1e9, ms
 
 
-def extras(sep, h, torch, np, cpu=True, large=True):
+EXTRA_SECTIONS = ("apply", "solve", "large", "backward", "dense", "batched")
+
+
+def extras(sep, h, torch, np, cpu=True, large=True, sections=EXTRA_SECTIONS):
     """Secondary figures: other apply shapes, config-1 solve, the differentiated solve, the
-    dense-input reduction (sytrd / orgtr), the batched solve."""
+    dense-input reduction (sytrd / orgtr), the batched solve (each a section that
+    `sections` selects)."""
     sys.path.insert(0, ROOT)
     ex = {}
-    for n, dt, name in ((2000, torch.float64, "apply_n2000_f64"),
+    for n, dt, name in () if "apply" not in sections else ((2000, torch.float64, "apply_n2000_f64"),
                         (2000, torch.float32, "apply_n2000_f32"),
                         (16384, torch.float32, "apply_n16384_f32")):
         gfs, ms = time_apply(sep, h, torch, n, 64, dt)
@@ -129,6 +133,20 @@ def extras(sep, h, torch, np, cpu=True, large=True):
             ex[name + "_over_published_8thread_cpu"] = round(gfs / PUBLISHED_N2000_8T[name], 1)
         log(f"[bench] {name}: {gfs:.1f} GFLOP/s ({ms:.3f} ms/step)")
 
+    if "solve" in sections:
+        solve_section(sep, h, torch, np, ex, cpu)
+    if large and "large" in sections:
+        large_section(sep, h, torch, ex)
+    if "backward" in sections:
+        backward_section(sep, h, torch, np, ex)
+    if "dense" in sections:
+        dense_section(sep, h, torch, np, ex)
+    if "batched" in sections:
+        batched_section(sep, h, torch, np, ex)
+    return ex
+
+
+def solve_section(sep, h, torch, np, ex, cpu):
     # config 1: README example solve, n = 2000, max_layers = 128
     d, e = sep.inputs.tridiag_linear(2000)
     H = sep.SymTridiagonal(torch.from_numpy(d).cuda(), torch.from_numpy(e).cuda())
@@ -155,9 +173,12 @@ def extras(sep, h, torch, np, cpu=True, large=True):
                                            "1 thread, chase + layer-major apply")
         log(f"[bench] solve n=2000 on the CPU port: {ex['solve_n2000_cpu_port_s']} s")
 
+
+
+def large_section(sep, h, torch, ex):
     # config 3: single large solve, n = 16384, max_layers = 64 (chase of block b + 1 beside
     # the apply of block b)
-    if large:
+    if True:
         n = 16384
         d, e = sep.inputs.tridiag_linear(n, seed=7)
         H = sep.SymTridiagonal(torch.from_numpy(d).cuda(), torch.from_numpy(e).cuda())
@@ -175,6 +196,9 @@ def extras(sep, h, torch, np, cpu=True, large=True):
         log(f"[bench] solve n=16384: {el:.2f} s, iter={it}, rotations={nrot}")
         del H, Q
 
+
+
+def backward_section(sep, h, torch, np, ex):
     # config 5: differentiated solve, n = 4096, d_i = i, e ~ U[0.1, 1)
     n = 4096
     d, e = sep.inputs.tridiag_linear(n, seed=1, e_low=0.1)
@@ -242,6 +266,8 @@ def extras(sep, h, torch, np, cpu=True, large=True):
         f"TFLOP/s on 4 sum n^3), ratio {bwd_b / fwd_b:.3f}")
     del dvb, evb, Qb, lbar, qbar
 
+
+def dense_section(sep, h, torch, np, ex):
     # dense symmetric input (no reference counterpart): reduction to tridiagonal and the
     # explicit orthogonal factor at n = 4096, A = (X + X^T) / 2 with X ~ N(0, 1)
     n = 4096
@@ -269,6 +295,8 @@ def extras(sep, h, torch, np, cpu=True, large=True):
     log(f"[bench] dense n=4096: sytrd {t_s * 1e3:.1f} ms ({100 * floor_s / t_s:.0f}% of its "
         f"HBM floor), orgtr {t_o * 1e3:.1f} ms ({ex['orgtr_n4096_tflops']} TFLOP/s on 4n^3/3)")
 
+
+def batched_section(sep, h, torch, np, ex):
     # config 4: 4096 uneven problems, even n in [32, 512], d_i = i + N(0, 0.1), e ~ U[0,1)
     B = 4096
     ns = sep.inputs.batched_orders(B)
@@ -320,7 +348,6 @@ def extras(sep, h, torch, np, cpu=True, large=True):
     ex["batched_over_loop"] = round(B / best / loop_rate, 1)
     log(f"[bench] batched B={B}: {best * 1e3:.1f} ms = {B / best:.0f} solves/s; per-problem "
         f"loop {loop_rate:.1f} solves/s; ratio {B / best / loop_rate:.0f}x")
-    return ex
 
 
 
@@ -338,6 +365,9 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-large", action="store_true",
                     help="skip the n = 16384 solve in the extras (about 40 s)")
+    ap.add_argument("--extras", default=",".join(EXTRA_SECTIONS),
+                    help="comma-separated extra sections to run (of "
+                         + ", ".join(EXTRA_SECTIONS) + "); profiling runs pick one")
     ap.add_argument("--cpu-rows", type=int, default=16384)
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--calibrate-copy-gib", type=float, default=0.0,
@@ -462,7 +492,8 @@ def main() -> int:
         del src, dst
 
     if not args.no_extra:
-        out["extra"] = extras(sep, h, torch, np, cpu=not args.no_cpu, large=not args.no_large)
+        out["extra"] = extras(sep, h, torch, np, cpu=not args.no_cpu, large=not args.no_large,
+                              sections=tuple(args.extras.split(",")))
     if cpu_extra:
         out.setdefault("extra", {}).update(cpu_extra)
 

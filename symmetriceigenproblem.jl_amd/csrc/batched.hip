@@ -316,8 +316,10 @@ __global__ void __launch_bounds__(kLanes)
                 const int* __restrict__ lt, const double* __restrict__ grid,
                 const int4* __restrict__ info, double* __restrict__ Q)
 {
-    for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x)
-        apply_item<KF, R>(work[wi], probs, lt, grid, info, Q);
+    for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+        const BWork wk = work[wi];
+        if (wk.prob >= 0) apply_item<KF, R>(wk, probs, lt, grid, info, Q);  // -1: padding
+    }
 }
 
 // max_layers, fused depth, rows per lane, and chase waves per SIMD (the chase kernel
@@ -372,8 +374,14 @@ int check_batch(int64_t B, const int64_t* n, const int64_t* off_d, const int64_t
     return SEP_OK;
 }
 
+// Work items by XCD: consecutive workgroups of a launch go to the device's XCDs in turn
+// (workgroup b to XCD b mod X), and each XCD has its own L2.  All row blocks of one problem
+// read the same staged (c, s) grid, so they are placed on one XCD (positions x, x + X,
+// x + 2X, ... of the list), where the grid is fetched from HBM once and then hits in L2.
+// Problems go, largest first, to the XCD with the least work so far (rows x columns); the
+// shorter queues are padded with empty items (prob = -1).
 void build_layout(int64_t B, const int64_t* n, const int64_t* off_d, const int64_t* off_q,
-                  const Config& cfg, Layout& L)
+                  const Config& cfg, int xcds, Layout& L)
 {
     L.probs.resize(B);
     L.work.clear();
@@ -397,8 +405,19 @@ void build_layout(int64_t B, const int64_t* n, const int64_t* off_d, const int64
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return n[a] > n[b]; });
     const int rows = kLanes * cfg.R;
-    for (int p : order)
-        for (int r0 = 0; r0 < (int)n[p]; r0 += rows) L.work.push_back(BWork{p, r0});
+    const int X = std::max(1, xcds);
+    std::vector<std::vector<BWork>> q(X);
+    std::vector<double> load(X, 0.0);
+    for (int p : order) {
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        for (int r0 = 0; r0 < (int)n[p]; r0 += rows) q[x].push_back(BWork{p, r0});
+        load[x] += (double)n[p] * (double)n[p];
+    }
+    size_t len = 0;
+    for (auto& v : q) len = std::max(len, v.size());
+    L.work.assign(len * X, BWork{-1, 0});
+    for (int x = 0; x < X; ++x)
+        for (size_t j = 0; j < q[x].size(); ++j) L.work[j * X + x] = q[x][j];
 }
 
 template <int KF, int R>
@@ -430,7 +449,7 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
                 int32_t* converged, int64_t* iters)
 {
     Layout L;
-    build_layout(B, n, off_d, off_q, cfg, L);
+    build_layout(B, n, off_d, off_q, cfg, h->num_xcds, L);
     if (int rc = ensure_batch_sync(h)) return rc;
     const size_t probs_b = sizeof(BProb) * (size_t)B;
     const size_t work_b = sizeof(BWork) * L.work.size();

@@ -242,6 +242,10 @@ int sep_create(sep_handle** out, int device)
     sep_handle* h = new sep_handle();
     h->device = device;
     h->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    int xcc = 0;
+    if (hipDeviceGetAttribute(&xcc, hipDeviceAttributeNumberOfXccs, device) == hipSuccess &&
+        xcc > 0)
+        h->num_xcds = xcc;
     if (const char* ap = std::getenv("SEP_APPLY")) {
         // apply geometry (unset: chosen by the number of rows and layers)
         static const struct { const char* name; int geom; } kGeoms[] = {

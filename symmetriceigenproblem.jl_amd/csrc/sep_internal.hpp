@@ -77,6 +77,7 @@ struct ApplyWork {
 struct sep_handle {
     int device = 0;
     int num_cus = 256;  // compute units of the device (hipDeviceProp_t::multiProcessorCount)
+    int num_xcds = 8;   // XCDs (each with its own L2); workgroups go to them in turn
     hipStream_t stream = nullptr;        // apply stream (user-visible)
     hipStream_t chase_stream = nullptr;  // chase stream (overlap with apply)
     bool own_stream = true;
