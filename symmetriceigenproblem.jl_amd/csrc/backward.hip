@@ -40,19 +40,45 @@ struct BwdProb {
 
 enum { kModeM = 0, kModeP = 1 };
 
+// Tile of this workgroup.  Workgroups go to the XCDs in turn (workgroup b to XCD b mod X,
+// in dispatch order b = x + y nt), and each XCD has its own L2.  For a square tile grid
+// whose size divides evenly, each XCD takes a contiguous run of 8 x 8 tile groups, in
+// dispatch order, so the workgroups resident on one XCD at a time share 8 row panels of A
+// and 8 column panels of B in its L2; otherwise the grid position is the tile.
+__device__ __forceinline__ void tile_of(int xcds, int& tx, int& ty)
+{
+    constexpr int G = 8;
+    const int nt = gridDim.x;
+    tx = blockIdx.x;
+    ty = blockIdx.y;
+    if (gridDim.z != 1 || (int)gridDim.y != nt || nt % G != 0 || xcds <= 0 ||
+        (nt * nt) % xcds != 0)
+        return;
+    const int b = blockIdx.x + blockIdx.y * nt;
+    const int per = nt * nt / xcds;
+    const int t = (b % xcds) * per + b / xcds;  // a bijection of 0 .. nt^2 - 1
+    const int g = t / (G * G), w = t % (G * G), gpr = nt / G;
+    ty = (g / gpr) * G + w / G;
+    tx = (g % gpr) * G + w % G;
+}
+
+// At most 256 registers per lane (accumulators included), so that two workgroups share a
+// CU and each SIMD has a second wave to issue MFMAs while one waits on memory or a barrier.
 template <int MODE>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads, 2)
     k_bwd_gemm(const BwdProb* __restrict__ probs, const double* __restrict__ lam,
                const double* __restrict__ lam_bar, const double* __restrict__ Q,
                const double* __restrict__ Qb, double* __restrict__ M,
-               double* __restrict__ part, int64_t part_ld)
+               double* __restrict__ part, int64_t part_ld, int xcds)
 {
     __shared__ double As[2][kBK][kLdsLd];
     __shared__ double Bs[2][kBK][kLdsLd];
 
     const BwdProb pr = probs[blockIdx.z];
     const int n = pr.n, ld = pr.ld;
-    const int i0 = blockIdx.x * kBT, j0 = blockIdx.y * kBT;
+    int tx, ty;
+    tile_of(xcds, tx, ty);
+    const int i0 = tx * kBT, j0 = ty * kBT;
     if (i0 >= n || j0 >= n) return;
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     const int wr = wid >> 1, wc = wid & 1;
@@ -62,8 +88,26 @@ __global__ void __launch_bounds__(kThreads)
     const int ldb = (MODE == kModeM) ? ld : n;
 
     double ra[8], rb[8];
-    // A[i][k] = Q[k + i ld] (first product, Q^T) or Q[i + k ld] (second product)
+    // A[i][k] = Q[k + i ld] (first product, Q^T) or Q[i + k ld] (second product).  Tiles
+    // entirely inside the matrix (all of them when n is a multiple of 128) load without
+    // per-element range tests.
+    const bool inner_ij = i0 + kBT <= n && j0 + kBT <= n;
     auto load_tile = [&](int k0) {
+        if (inner_ij && k0 + kBK <= n) {
+            if constexpr (MODE == kModeM) {
+                const double* src = Qp + (int64_t)(i0 + (t >> 1)) * ld + k0 + (t & 1) * 8;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) ra[u] = src[u];
+            } else {
+                const double* src = Qp + (int64_t)(k0 + (t >> 4)) * ld + i0 + (t & 15) * 8;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) ra[u] = src[u];
+            }
+            const double* src = Bp + (int64_t)(j0 + (t >> 1)) * ldb + k0 + (t & 1) * 8;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) rb[u] = src[u];
+            return;
+        }
         if constexpr (MODE == kModeM) {
             const int i = i0 + (t >> 1), kb = k0 + (t & 1) * 8;
             const double* src = Qp + (int64_t)i * ld + kb;
@@ -189,7 +233,7 @@ __global__ void __launch_bounds__(kThreads)
             if (i < n) {
                 const double* a = red + (size_t)t * 3;
                 const double* b = red + ((size_t)kBT + t) * 3;
-                double* dst = part + (int64_t)blockIdx.y * 3 * part_ld + pr.off_d + i;
+                double* dst = part + (int64_t)ty * 3 * part_ld + pr.off_d + i;
                 dst[0] = a[0] + b[0];
                 dst[part_ld] = a[1] + b[1];
                 dst[2 * part_ld] = a[2] + b[2];
@@ -250,10 +294,10 @@ int run_backward(sep_handle* h, const std::vector<BwdProb>& probs, int nmax, int
     double* part = h->bwd_part.as<double>();
     const dim3 grid(nct, nct, (unsigned)B);
     hipLaunchKernelGGL((k_bwd_gemm<kModeM>), grid, dim3(kThreads), 0, h->stream, dp, lam,
-                       lam_bar, Q, Q_bar, Mb, part, part_ld);
+                       lam_bar, Q, Q_bar, Mb, part, part_ld, h->num_xcds);
     SEP_CHECK_LAUNCH();
     hipLaunchKernelGGL((k_bwd_gemm<kModeP>), grid, dim3(kThreads), 0, h->stream, dp, lam,
-                       lam_bar, Q, Q_bar, Mb, part, part_ld);
+                       lam_bar, Q, Q_bar, Mb, part, part_ld, h->num_xcds);
     SEP_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_bwd_reduce, dim3((nmax + 255) / 256, (unsigned)B), dim3(256), 0,
                        h->stream, dp, part, part_ld, d_bar, e_bar);
