@@ -15,7 +15,8 @@
 //                       partial sums and stores them per column tile (no atomics)
 //   k_bwd_reduce:       sums the column-tile partials in a fixed order -> d_bar, e_bar
 // Block tile 128 x 128, k step 16, 256 threads = 2 x 2 waves of 64 x 64 (4 x 4 MFMA tiles of
-// 16 x 16, four f64 accumulators per lane each), double-buffered LDS, one barrier per k step.
+// 16 x 16, four f64 accumulators per lane each), double-buffered LDS, one barrier per k step;
+// batches whose orders are all at most 512 use 64 x 64 block tiles (waves of 32 x 32).
 // Problems are described by a table, so the single and the batched entry points share the
 // kernels (grid z = problem; tiles beyond a problem's order exit at once).
 #include "sep_internal.hpp"
@@ -27,8 +28,8 @@ using f64x4 = double __attribute__((ext_vector_type(4)));
 
 constexpr int kBT = 128;      // block tile (rows = columns)
 constexpr int kBK = 16;       // k step
-constexpr int kLdsLd = kBT + 4;
 constexpr int kThreads = 256;
+constexpr int kSmallTileMaxN = 512;  // largest order for which 64-tiles are used
 
 struct BwdProb {
     int n;
@@ -64,21 +65,29 @@ __device__ __forceinline__ void tile_of(int xcds, int& tx, int& ty)
 
 // At most 256 registers per lane (accumulators included), so that two workgroups share a
 // CU and each SIMD has a second wave to issue MFMAs while one waits on memory or a barrier.
-template <int MODE>
+// BT is the block tile: 128 (2 x 2 waves of 64 x 64) for large orders, 64 (2 x 2 waves of
+// 32 x 32) for batches of small problems, where a 128 tile would mostly compute padding.
+template <int MODE, int BT>
 __global__ void __launch_bounds__(kThreads, 2)
     k_bwd_gemm(const BwdProb* __restrict__ probs, const double* __restrict__ lam,
                const double* __restrict__ lam_bar, const double* __restrict__ Q,
                const double* __restrict__ Qb, double* __restrict__ M,
                double* __restrict__ part, int64_t part_ld, int xcds)
 {
-    __shared__ double As[2][kBK][kLdsLd];
-    __shared__ double Bs[2][kBK][kLdsLd];
+    constexpr int LDS_LD = BT + 4;
+    constexpr int WT = BT / 32;          // 16 x 16 MFMA tiles per wave and dimension
+    constexpr int TPI = kThreads / BT;   // threads per A row (first product) / B row
+    constexpr int EPT = kBK / TPI;       // elements each of them loads
+    constexpr int EPK = BT / 16;         // second product: elements per thread along i
+    static_assert(EPT * TPI == kBK && EPK * 16 == BT, "tile geometry");
+    __shared__ double As[2][kBK][LDS_LD];
+    __shared__ double Bs[2][kBK][LDS_LD];
 
     const BwdProb pr = probs[blockIdx.z];
     const int n = pr.n, ld = pr.ld;
     int tx, ty;
     tile_of(xcds, tx, ty);
-    const int i0 = tx * kBT, j0 = ty * kBT;
+    const int i0 = tx * BT, j0 = ty * BT;
     if (i0 >= n || j0 >= n) return;
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     const int wr = wid >> 1, wc = wid & 1;
@@ -87,84 +96,75 @@ __global__ void __launch_bounds__(kThreads, 2)
     const double* __restrict__ Bp = (MODE == kModeM) ? Qb + pr.off_q : M + pr.off_m;
     const int ldb = (MODE == kModeM) ? ld : n;
 
-    double ra[8], rb[8];
-    // A[i][k] = Q[k + i ld] (first product, Q^T) or Q[i + k ld] (second product).  Tiles
-    // entirely inside the matrix (all of them when n is a multiple of 128) load without
-    // per-element range tests.
-    const bool inner_ij = i0 + kBT <= n && j0 + kBT <= n;
+    // this thread's A and B elements: A[i][k] = Q[k + i ld] (first product, Q^T) or
+    // Q[i + k ld] (second product), B[k][j] = Bp[k + j ldb]
+    const int ai = (MODE == kModeM) ? t / TPI : (t & 15) * EPK;
+    const int ak = (MODE == kModeM) ? (t % TPI) * EPT : t >> 4;
+    const int bj = t / TPI, bk = (t % TPI) * EPT;
+    constexpr int AE = (MODE == kModeM) ? EPT : EPK;
+    double ra[AE], rb[EPT];
+    // Tiles entirely inside the matrix (all of them when n is a multiple of BT) load
+    // without per-element range tests.
+    const bool inner_ij = i0 + BT <= n && j0 + BT <= n;
     auto load_tile = [&](int k0) {
+        const double* srcb = Bp + (int64_t)(j0 + bj) * ldb + k0 + bk;
+        const double* srca = (MODE == kModeM) ? Qp + (int64_t)(i0 + ai) * ld + k0 + ak
+                                              : Qp + (int64_t)(k0 + ak) * ld + i0 + ai;
         if (inner_ij && k0 + kBK <= n) {
-            if constexpr (MODE == kModeM) {
-                const double* src = Qp + (int64_t)(i0 + (t >> 1)) * ld + k0 + (t & 1) * 8;
 #pragma unroll
-                for (int u = 0; u < 8; ++u) ra[u] = src[u];
-            } else {
-                const double* src = Qp + (int64_t)(k0 + (t >> 4)) * ld + i0 + (t & 15) * 8;
+            for (int u = 0; u < AE; ++u) ra[u] = srca[u];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) ra[u] = src[u];
-            }
-            const double* src = Bp + (int64_t)(j0 + (t >> 1)) * ldb + k0 + (t & 1) * 8;
-#pragma unroll
-            for (int u = 0; u < 8; ++u) rb[u] = src[u];
+            for (int u = 0; u < EPT; ++u) rb[u] = srcb[u];
             return;
         }
-        if constexpr (MODE == kModeM) {
-            const int i = i0 + (t >> 1), kb = k0 + (t & 1) * 8;
-            const double* src = Qp + (int64_t)i * ld + kb;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) ra[u] = (i < n && kb + u < n) ? src[u] : 0.0;
-        } else {
-            const int k = k0 + (t >> 4), ib = i0 + (t & 15) * 8;
-            const double* src = Qp + (int64_t)k * ld + ib;
-#pragma unroll
-            for (int u = 0; u < 8; ++u) ra[u] = (k < n && ib + u < n) ? src[u] : 0.0;
+        for (int u = 0; u < AE; ++u) {
+            const bool in = (MODE == kModeM) ? (i0 + ai < n && k0 + ak + u < n)
+                                             : (k0 + ak < n && i0 + ai + u < n);
+            ra[u] = in ? srca[u] : 0.0;
         }
-        const int j = j0 + (t >> 1), kb = k0 + (t & 1) * 8;
-        const double* src = Bp + (int64_t)j * ldb + kb;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) rb[u] = (j < n && kb + u < n) ? src[u] : 0.0;
+        for (int u = 0; u < EPT; ++u) rb[u] = (j0 + bj < n && k0 + bk + u < n) ? srcb[u] : 0.0;
     };
     auto store_tile = [&](int buf) {
-        if constexpr (MODE == kModeM) {
-            const int i = t >> 1, kb = (t & 1) * 8;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) As[buf][kb + u][i] = ra[u];
-        } else {
-            const int k = t >> 4, ib = (t & 15) * 8;
-#pragma unroll
-            for (int u = 0; u < 8; ++u) As[buf][k][ib + u] = ra[u];
+        for (int u = 0; u < AE; ++u) {
+            if constexpr (MODE == kModeM)
+                As[buf][ak + u][ai] = ra[u];
+            else
+                As[buf][ak][ai + u] = ra[u];
         }
-        const int j = t >> 1, kb = (t & 1) * 8;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) Bs[buf][kb + u][j] = rb[u];
+        for (int u = 0; u < EPT; ++u) Bs[buf][bk + u][bj] = rb[u];
     };
 
-    f64x4 acc[4][4];
+    f64x4 acc[WT][WT];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < WT; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+        for (int b = 0; b < WT; ++b) acc[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
 
     const int nk = (n + kBK - 1) / kBK;
     load_tile(0);
     store_tile(0);
     __syncthreads();
     const int fr = lane & 15, fk = lane >> 4;  // fragment row / k of this lane
+    constexpr int WS = BT / 2;                 // wave tile
     for (int kt = 0; kt < nk; ++kt) {
         const int cur = kt & 1;
         if (kt + 1 < nk) load_tile((kt + 1) * kBK);
 #pragma unroll
         for (int ks = 0; ks < kBK / 4; ++ks) {
             const int kr = ks * 4 + fk;
-            double a[4], b[4];
+            double a[WT], b[WT];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) a[r] = As[cur][kr][wr * 64 + r * 16 + fr];
+            for (int r = 0; r < WT; ++r) a[r] = As[cur][kr][wr * WS + r * 16 + fr];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) b[c] = Bs[cur][kr][wc * 64 + c * 16 + fr];
+            for (int c = 0; c < WT; ++c) b[c] = Bs[cur][kr][wc * WS + c * 16 + fr];
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
+            for (int r = 0; r < WT; ++r)
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
+                for (int c = 0; c < WT; ++c)
                     acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
         }
         if (kt + 1 < nk) store_tile(cur ^ 1);
@@ -178,15 +178,15 @@ __global__ void __launch_bounds__(kThreads, 2)
         const double* __restrict__ lbp = lam_bar + pr.off_d;
         double* __restrict__ Mp = M + pr.off_m;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int j = j0 + wc * 64 + c * 16 + fr;
+        for (int c = 0; c < WT; ++c) {
+            const int j = j0 + wc * WS + c * 16 + fr;
             if (j >= n) continue;
             const double lj = lp[j];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
+            for (int r = 0; r < WT; ++r) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int i = i0 + wr * 64 + r * 16 + fk + 4 * v;
+                    const int i = i0 + wr * WS + r * 16 + fk + 4 * v;
                     if (i >= n) continue;
                     Mp[i + (int64_t)j * n] = (i == j) ? lbp[i] : acc[r][c][v] / (lj - lp[i]);
                 }
@@ -194,17 +194,18 @@ __global__ void __launch_bounds__(kThreads, 2)
         }
     } else {
         // the LDS tiles are free after the last barrier: reuse As for the wave partials
-        double* red = &As[0][0][0];  // [2 wc][128 rows][3]
+        static_assert(2 * kBK * LDS_LD >= 2 * BT * 3, "partials fit in the A tiles");
+        double* red = &As[0][0][0];  // [2 wc][BT rows][3]
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < WT; ++r) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int il = wr * 64 + r * 16 + fk + 4 * v;
+                const int il = wr * WS + r * 16 + fk + 4 * v;
                 const int i = i0 + il;
                 double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const int j = j0 + wc * 64 + c * 16 + fr;
+                for (int c = 0; c < WT; ++c) {
+                    const int j = j0 + wc * WS + c * 16 + fr;
                     if (i < n && j < n) {
                         const double p = acc[r][c][v];
                         const double* qc = Qp + (int64_t)j * ld;
@@ -220,7 +221,7 @@ __global__ void __launch_bounds__(kThreads, 2)
                     s2 += __shfl_xor(s2, off, 64);
                 }
                 if (fr == 0) {
-                    double* o = red + ((size_t)wc * kBT + il) * 3;
+                    double* o = red + ((size_t)wc * BT + il) * 3;
                     o[0] = s0;
                     o[1] = s1;
                     o[2] = s2;
@@ -228,11 +229,11 @@ __global__ void __launch_bounds__(kThreads, 2)
             }
         }
         __syncthreads();
-        if (t < kBT) {
+        if (t < BT) {
             const int i = i0 + t;
             if (i < n) {
                 const double* a = red + (size_t)t * 3;
-                const double* b = red + ((size_t)kBT + t) * 3;
+                const double* b = red + ((size_t)BT + t) * 3;
                 double* dst = part + (int64_t)ty * 3 * part_ld + pr.off_d + i;
                 dst[0] = a[0] + b[0];
                 dst[part_ld] = a[1] + b[1];
@@ -245,12 +246,12 @@ __global__ void __launch_bounds__(kThreads, 2)
 // d_bar_k = sum_ct S0[ct][k];  e_bar_k = sum_ct (S1[ct][k] + S2[ct][k+1])
 __global__ void __launch_bounds__(256)
     k_bwd_reduce(const BwdProb* __restrict__ probs, const double* __restrict__ part,
-                 int64_t part_ld, double* __restrict__ d_bar, double* __restrict__ e_bar)
+                 int64_t part_ld, int bt, double* __restrict__ d_bar, double* __restrict__ e_bar)
 {
     const BwdProb pr = probs[blockIdx.y];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= pr.n) return;
-    const int nct = (pr.n + kBT - 1) / kBT;
+    const int nct = (pr.n + bt - 1) / bt;
     const int64_t o = pr.off_d + k;
     double d = 0.0, e = 0.0;
     for (int ct = 0; ct < nct; ++ct) {
@@ -272,7 +273,9 @@ int run_backward(sep_handle* h, const std::vector<BwdProb>& probs, int nmax, int
         set_error("eig_backward: at most 65535 problems per call (got %lld)", (long long)B);
         return SEP_ERR_ARG;
     }
-    const int nct = (nmax + kBT - 1) / kBT;
+    // 64-tiles when every order is small (the 128 tile's padding would dominate)
+    const int bt = nmax <= kSmallTileMaxN ? 64 : kBT;
+    const int nct = (nmax + bt - 1) / bt;
     const size_t meta_bytes = sizeof(BwdProb) * (size_t)B;
     if (int rc = h->bwd_meta.reserve(meta_bytes)) return rc;
     if (int rc = h->bwd_M.reserve(sizeof(double) * (size_t)tot_m)) return rc;
@@ -293,14 +296,22 @@ int run_backward(sep_handle* h, const std::vector<BwdProb>& probs, int nmax, int
     double* Mb = h->bwd_M.as<double>();
     double* part = h->bwd_part.as<double>();
     const dim3 grid(nct, nct, (unsigned)B);
-    hipLaunchKernelGGL((k_bwd_gemm<kModeM>), grid, dim3(kThreads), 0, h->stream, dp, lam,
-                       lam_bar, Q, Q_bar, Mb, part, part_ld, h->num_xcds);
-    SEP_CHECK_LAUNCH();
-    hipLaunchKernelGGL((k_bwd_gemm<kModeP>), grid, dim3(kThreads), 0, h->stream, dp, lam,
-                       lam_bar, Q, Q_bar, Mb, part, part_ld, h->num_xcds);
+    if (bt == 64) {
+        hipLaunchKernelGGL((k_bwd_gemm<kModeM, 64>), grid, dim3(kThreads), 0, h->stream, dp,
+                           lam, lam_bar, Q, Q_bar, Mb, part, part_ld, h->num_xcds);
+        SEP_CHECK_LAUNCH();
+        hipLaunchKernelGGL((k_bwd_gemm<kModeP, 64>), grid, dim3(kThreads), 0, h->stream, dp,
+                           lam, lam_bar, Q, Q_bar, Mb, part, part_ld, h->num_xcds);
+    } else {
+        hipLaunchKernelGGL((k_bwd_gemm<kModeM, kBT>), grid, dim3(kThreads), 0, h->stream, dp,
+                           lam, lam_bar, Q, Q_bar, Mb, part, part_ld, h->num_xcds);
+        SEP_CHECK_LAUNCH();
+        hipLaunchKernelGGL((k_bwd_gemm<kModeP, kBT>), grid, dim3(kThreads), 0, h->stream, dp,
+                           lam, lam_bar, Q, Q_bar, Mb, part, part_ld, h->num_xcds);
+    }
     SEP_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_bwd_reduce, dim3((nmax + 255) / 256, (unsigned)B), dim3(256), 0,
-                       h->stream, dp, part, part_ld, d_bar, e_bar);
+                       h->stream, dp, part, part_ld, bt, d_bar, e_bar);
     SEP_CHECK_LAUNCH();
     return SEP_OK;
 }
