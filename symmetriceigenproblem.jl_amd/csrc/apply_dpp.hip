@@ -34,9 +34,12 @@
 //    instructions): the hardware range check returns 0 for columns past n and drops
 //    stores outside the matrix, so every wave runs every interval on one code path with
 //    no per-lane address arithmetic.
-//  * Only the middle waves load coefficients (for every wave, one interval ahead, staged
-//    through LDS); the first wave issues only Q loads and the last only Q stores, so no
-//    wave ever has to wait for a memory operation it issued for a later interval.
+//  * Only the middle waves load coefficients: each its own rows straight into registers
+//    (one row per step, for the next interval), and the first and last waves' rows one
+//    interval ahead, staged through LDS.  The first wave issues only Q loads and the last
+//    only Q stores, so no wave ever has to wait for a memory operation it issued for a
+//    later interval.  (Staging every wave's rows through LDS costs 2% at m = n = 16384,
+//    k = 64: the extra LDS reads raise the power draw that sets the clock.)
 //  * Rotations are applied as three shears, a += p b; b += q a; a += p b: three in-place
 //    FMAs per rotation and row instead of two multiplies and two FMAs.  For a rotation
 //    (c^2 + s^2 = 1) p = s~/(1 + |c|) = tan(phi/2), q = -s~, with the sign of c tracked
@@ -432,23 +435,25 @@ __device__ __forceinline__ void col_store(T* p, int col, int n, uint32_t voff, T
 template <typename T, class Gm, bool FIRST, bool LAST, bool GENERAL, int D>
 __device__ __forceinline__ void interval(T* At, uint32_t voff, int64_t lda, int n,
                                          T (&win)[Gm::W], T (&pf)[FIRST ? Gm::PFD : 1][Gm::S],
-                                         const T* cbw, T sg,
+                                         const T* cbw, T sg, const T (&cfo)[Gm::NR],
+                                         T (&cfn)[Gm::NR], const T* __restrict__ tn,
                                          const T* lds_in, T* lds_out, int t0)
 {
     constexpr int S = Gm::S, W = Gm::W, KG = Gm::KG, PFD = Gm::PFD, NR = Gm::NR;
     constexpr int SPR = Gm::SPR;
-    // Reads of LDS (this wave's input columns and coefficient rows) are issued LA steps
-    // before the step that uses them.  Issuing them all at the start of the interval
+    // Reads of LDS (this wave's input columns, and the first and last waves' coefficient
+    // rows) are issued LA steps before the step that uses them.  Issuing them all at the start of the interval
     // makes every wave of the workgroup queue its whole interval's reads at the LDS right
     // after the barrier, so the last wave in the queue waits for all of them before its
     // first FMA.
     constexpr int LA = 2;
+    constexpr bool OWN = !FIRST && !LAST;  // coefficient rows in registers
     T in[FIRST ? 1 : S];
-    T cf[NR];
+    T cfl[OWN ? 1 : NR];
     auto read_in = [&](auto sc) {
         constexpr int s = decltype(sc)::value;
         if constexpr (!FIRST && s < S) in[s] = lds_in[s * kLanes];
-        if constexpr (s < S && s % SPR == 0) cf[s / SPR] = cbw[(s / SPR) * 16];
+        if constexpr (!OWN && s < S && s % SPR == 0) cfl[s / SPR] = cbw[(s / SPR) * 16];
     };
     Steps<LA, 0>::run(read_in);
     // first wave: prefetch slot D holds this interval's columns (loaded PFD intervals
@@ -467,6 +472,13 @@ __device__ __forceinline__ void interval(T* At, uint32_t voff, int64_t lda, int 
         // the barrier early and leaves the younger one to run alone).
         if constexpr (s % (S / 4) == 0) __builtin_amdgcn_s_setprio(3 - s / (S / 4));
         read_in(std::integral_constant<int, s + LA>{});
+        // middle waves: row s of the next interval's coefficients (cfo holds this one's)
+        if constexpr (OWN && s < NR) cfn[s] = tn[s * 16];
+        T cfr;
+        if constexpr (OWN)
+            cfr = cfo[row];
+        else
+            cfr = cfl[row];
         if constexpr (FIRST) {
             // the column enters the pass with its final scale (see k_colscale)
             win[(s + 1) % W] = pf[D][s];
@@ -477,9 +489,9 @@ __device__ __forceinline__ void interval(T* At, uint32_t voff, int64_t lda, int 
             win[(s + 1) % W] = in[s];
         }
         if constexpr (GENERAL)
-            step_general<T, KG, s, base>(win, cf[row]);
+            step_general<T, KG, s, base>(win, cfr);
         else
-            step_shear<T, KG, s, base>(win, cf[row]);
+            step_shear<T, KG, s, base>(win, cfr);
         T out = win[(s + 2) % W];
         if constexpr (LAST) {
             col_store<T>(pq, o0 + s, n, voff, out);
@@ -491,11 +503,11 @@ __device__ __forceinline__ void interval(T* At, uint32_t voff, int64_t lda, int 
     });
 }
 
-// LDS layout of one interval's coefficient block: rows [g][r][16] for every wave, then
-// the first wave's 16-lane input-scale row.
+// LDS layout of one interval's coefficient block: rows [2][r][16] of the first and the
+// last wave, then the first wave's 16-lane input-scale row.
 template <class Gm>
 struct CoefBlock {
-    static constexpr int ROWS = Gm::G * Gm::NR * 16;
+    static constexpr int ROWS = 2 * Gm::NR * 16;
     static constexpr int SIZE = ROWS + 16;
     static constexpr int LOADERS = Gm::G > 2 ? Gm::G - 2 : 1;          // middle waves
     static constexpr int PER = (SIZE + LOADERS * kLanes - 1) / (LOADERS * kLanes);
@@ -514,7 +526,7 @@ __device__ __forceinline__ void block_offsets(int (&off)[CoefBlock<Gm>::PER], in
         const int q = loader * kLanes + lane + j * CB::LOADERS * kLanes;
         int gq = 0, r = 0, c = 0;
         if (q < CB::ROWS) {
-            gq = q / (NR * 16);
+            gq = q < NR * 16 ? 0 : G - 1;
             r = (q / 16) % NR;
             c = q % 16;
         } else if (q < CB::SIZE) {
@@ -525,10 +537,11 @@ __device__ __forceinline__ void block_offsets(int (&off)[CoefBlock<Gm>::PER], in
     }
 }
 
-// The middle waves fetch the next interval's coefficient block (every wave's rows and
-// the input scales) from the table into registers at the start of an interval and store
-// it to LDS at the end; they issue no other global memory operations, so waiting for
-// these loads never drains the first wave's Q prefetch or the last wave's stores.
+// The middle waves fetch the next interval's coefficient block (the first and last
+// waves' rows and the input scales) from the table into registers at the start of an
+// interval and store it to LDS at the end; their other global memory operations are
+// loads of their own coefficient rows, so waiting for these loads never drains the first
+// wave's Q prefetch or the last wave's stores.
 template <typename T, class Gm>
 __device__ __forceinline__ void fetch_block(T (&v)[CoefBlock<Gm>::PER],
                                             const int (&off)[CoefBlock<Gm>::PER],
@@ -594,6 +607,15 @@ __device__ __forceinline__ void wave(T* __restrict__ A, int64_t lda, int m, int 
             ring[g][1][s][lane] = 0.0;
         }
     }
+    // a middle wave's own coefficient rows: interval b in cfb[b & 1], interval b + 1
+    // loaded into the other buffer one row per step while b runs
+    constexpr int NC = Gm::NC;
+    const T* __restrict__ tw = tab + (int64_t)(g * Up + g * (1 - S) + S + PADU) * NC + cl;
+    T cfb[2][NR];
+    if constexpr (LOADER) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) cfb[0][r] = tw[r * 16];
+    }
     T blk[CB::PER];
     int off[CB::PER];
     if constexpr (LOADER) {  // block of interval 0
@@ -602,10 +624,12 @@ __device__ __forceinline__ void wave(T* __restrict__ A, int64_t lda, int m, int 
         store_block<T, Gm>(cbuf[0], blk, loader, lane);
     }
     lds_barrier();
-    // the interval loop is unrolled by the prefetch depth so that prefetch slot D is
-    // consumed and refilled in place (no register shuffling of in-flight loads)
-    for (int b0 = 0; b0 < NI; b0 += PFD) {
-        Steps<PFD, 0>::run([&](auto dc) {
+    // The interval loop is unrolled by the prefetch depth (rounded up to even) so that
+    // prefetch slot D % PFD and coefficient buffer D & 1 are compile-time registers,
+    // consumed and refilled in place (no register shuffling of in-flight loads).
+    constexpr int U = PFD % 2 == 0 ? PFD : 2 * PFD;
+    for (int b0 = 0; b0 < NI; b0 += U) {
+        Steps<U, 0>::run([&](auto dc) {
             constexpr int D = decltype(dc)::value;
             const int b = b0 + D;
             if (b >= NI) return;
@@ -618,12 +642,15 @@ __device__ __forceinline__ void wave(T* __restrict__ A, int64_t lda, int m, int 
             // (range check) and keep the window in the same registers on every path.
             const int t0 = (b - g) * S + g;
             const T* cb = cbuf[b & 1];
-            const T* cbw = cb + g * NR * 16 + cl;  // this wave's coefficient rows
+            const T* cbw = cb + (LAST ? NR * 16 : 0) + cl;  // first / last wave's rows
             const T sg = FIRST ? cb[CB::ROWS + cl] : 0.0;  // input scales
+            // next interval's rows (the last interval re-reads its own: in range, unused)
+            const int64_t o = (int64_t)min(b + 1, NI - 1) * (S * NC);
             const T* lds_in = FIRST ? nullptr : &ring[g > 0 ? g - 1 : 0][(b - 1) & 1][0][lane];
             T* lds_out = LAST ? nullptr : &ring[g][b & 1][0][lane];
-            interval<T, Gm, FIRST, LAST, GENERAL, D>(At, voff, lda, n, win, pf, cbw, sg, lds_in,
-                                                     lds_out, t0);
+            interval<T, Gm, FIRST, LAST, GENERAL, D % PFD>(At, voff, lda, n, win, pf, cbw, sg,
+                                                           cfb[D & 1], cfb[(D + 1) & 1], tw + o,
+                                                           lds_in, lds_out, t0);
             if constexpr (LOADER) {
                 if (b + 1 < NI) store_block<T, Gm>(cbuf[(b + 1) & 1], blk, loader, lane);
             }
