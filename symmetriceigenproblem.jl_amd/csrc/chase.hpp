@@ -473,6 +473,10 @@ template <typename T, bool kSolo = true>
 __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int lane)
 {
     __shared__ T chunk_in[2][kWave];  // dv[i+1], ev[i+1] of the chunk's steps (!kSolo)
+    // kSolo: step j's (dv[i+1], ev[i+1]) at solo_in[j] (two pad entries for the read-ahead)
+    // and its outputs (nrm, B11'', c, s) at solo_out[j]
+    __shared__ T solo_in[kWave + 2][2];
+    __shared__ T solo_out[kWave][4];
     T* __restrict__ dv = P.dv;
     T* __restrict__ ev = P.ev;
     int* __restrict__ lt = P.lt;
@@ -529,11 +533,14 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
             const int Lt = nL;
             // this chunk's inputs are complete here (one wait per chunk, not per step)
             asm volatile("" ::"v"(D), "v"(E));
+            // read back with one broadcast LDS read per value (a wave's LDS operations
+            // complete in order, so the previous chunk's reads are done)
             if constexpr (!kSolo) {
-                // read back with one broadcast LDS read per value (a wave's LDS operations
-                // complete in order, so the previous chunk's reads are done)
                 chunk_in[0][lane] = D;
                 chunk_in[1][lane] = E;
+            } else {
+                solo_in[lane][0] = D;
+                solo_in[lane][1] = E;
             }
             {  // prefetch next chunk
                 const int i = i0 + kWave + lane;
@@ -554,8 +561,26 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
             uint32_t acc = 0;
             auto run = [&](auto deferred) {
                 uint64_t jm = 1;  // lane mask of step j
+                // kSolo: the inputs of steps j + 1 and j + 2, read from LDS ahead of use
+                T a1D = T(0), a1E = T(0), a2D = T(0), a2E = T(0);
+                if constexpr (kSolo) {
+                    a1D = solo_in[0][0];
+                    a1E = solo_in[0][1];
+                    a2D = solo_in[1][0];
+                    a2E = solo_in[1][1];
+                }
                 auto step = [&](int j) {
-                    const T B22 = kSolo ? rdlane(D, j) : chunk_in[0][j];
+                    T B22, B23n;
+                    if constexpr (kSolo) {
+                        B22 = a1D;
+                        B23n = a1E;
+                        a1D = a2D;
+                        a1E = a2E;
+                        a2D = solo_in[j + 2][0];
+                        a2E = solo_in[j + 2][1];
+                    } else {
+                        B22 = chunk_in[0][j];
+                    }
                     const Rot<T> G = step_rotation(deferred, p1, bulge, acc);
                     const T c = G.c, s = G.s;
                     const T B11 = d_cur, B21 = e_cur;
@@ -566,13 +591,21 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                     const T B11pp = __builtin_fma(B11p, c, B12p * s);
                     const T B12pp = __builtin_fma(B11p, -s, B12p * c);
                     const T B22pp = __builtin_fma(B21p, -s, B22p * c);
-                    capture_lane(oR, oD, oC, oS, G.r, B11pp, c, s, jm);
-                    jm <<= 1;
+                    if constexpr (kSolo) {
+                        // every lane writes the same values (the LDS pipe, not the VALU)
+                        solo_out[j][0] = G.r;
+                        solo_out[j][1] = B11pp;
+                        solo_out[j][2] = c;
+                        solo_out[j][3] = s;
+                    } else {
+                        capture_lane(oR, oD, oC, oS, G.r, B11pp, c, s, jm);
+                        jm <<= 1;
+                    }
                     p1 = B12pp;
                     d_cur = B22pp;
                     // B23 = ev[i+1]; the last step of the sweep has none: its prefetch slot
                     // holds 0, and the bulge / e_cur it then produces are never read
-                    const T B23 = kSolo ? rdlane(E, j) : chunk_in[1][j];
+                    const T B23 = kSolo ? B23n : chunk_in[1][j];
                     bulge = s * B23;
                     e_cur = c * B23;
                 };
@@ -597,6 +630,12 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                 }
             } else {
                 run(std::false_type{});
+            }
+            if constexpr (kSolo) {
+                oR = solo_out[lane][0];
+                oD = solo_out[lane][1];
+                oC = solo_out[lane][2];
+                oS = solo_out[lane][3];
             }
             // coalesced stores of the chunk; the next chunk was prefetched from entries
             // these stores do not touch (dv/ev beyond i0 + 62, lt beyond i0 + 62)
