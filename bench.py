@@ -162,6 +162,22 @@ def solve_section(sep, h, torch, np, ex, cpu):
     # the reference README's qr_algorithm! at n = 2000 (its own CPU, unseeded input): 0.258 s
     ex["solve_n2000_over_published_cpu"] = round(0.258 / ex["solve_n2000_s"], 2)
     log(f"[bench] solve n=2000: {ex['solve_n2000_s']} s, iter={it}, rotations={nrot}")
+    # the same solve with the opt-in fast chase rotation (results equal to rounding)
+    H = sep.SymTridiagonal(torch.from_numpy(d).cuda(), torch.from_numpy(e).cuda())
+    Q = torch.eye(2000, dtype=torch.float64, device="cuda").t()
+    h.set_chase_rotations("fast")
+    try:
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, conv, it, nrot = sep.qr_algorithm_(H, Q, handle=h, return_rotations=True)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+    finally:
+        h.set_chase_rotations("exact")
+    ex["solve_n2000_fast_rotations_s"] = round(el, 4)
+    ex["solve_n2000_fast_rotations_iter"] = it
+    ex["solve_n2000_fast_rotations_over_published_cpu"] = round(0.258 / el, 2)
+    log(f"[bench] solve n=2000, fast rotations: {el:.4f} s, iter={it}, rotations={nrot}")
     if cpu:
         import oracle
         dv, ev = d.copy(), e.copy()

@@ -56,7 +56,7 @@ EXPORTS = (
     "sep_qr_algorithm_f64", "sep_qr_algorithm_f32", "sep_qr_algorithm_f64_dev",
     "sep_qr_algorithm_f32_dev",
     "sep_qr_batched_f64_dev", "sep_tune_batched_f64", "sep_set_batched_config",
-    "sep_set_odd_order",
+    "sep_set_odd_order", "sep_set_chase_rotations",
     "sep_eig_backward_f64_dev", "sep_eig_backward_batched_f64_dev",
     "sep_sytrd_f64_dev", "sep_orgtr_f64_dev",
     "sep_sort_eigen_f64_dev", "sep_sort_eigen_f32_dev",
@@ -113,6 +113,7 @@ def _declare(L: C.CDLL) -> None:
     L.sep_tune_batched_f64.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P]
     L.sep_set_batched_config.argtypes = [_P, C.c_int32, C.c_int32]
     L.sep_set_odd_order.argtypes = [_P, C.c_int32]
+    L.sep_set_chase_rotations.argtypes = [_P, C.c_int32]
     L.sep_eig_backward_f64_dev.argtypes = [_P, _I64, _P, _P, _I64, _P, _P, _P, _P]
     L.sep_eig_backward_batched_f64_dev.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P]
     L.sep_sytrd_f64_dev.argtypes = [_P, _I64, _P, _I64, _P, _P, _P]
@@ -191,6 +192,16 @@ class Handle:
         """Accept odd orders in this handle's solves (an extension: the reference asserts
         iseven(n) as a limitation of its wave kernels, src/bulge_chasing.jl:155)."""
         _check(lib().sep_set_odd_order(self._h, int(bool(allow))))
+
+    def set_chase_rotations(self, mode: str = "exact") -> None:
+        """Rotation arithmetic of this handle's Float64 solves: "exact" (the default, the
+        reference's givensAlgorithm, bitwise-equal results) or "fast" (one reciprocal
+        square root per chase rotation, results equal to rounding; no counterpart in the
+        reference).  Float32 solves always use the exact rotation."""
+        modes = {"exact": 0, "fast": 1}
+        if mode not in modes:
+            raise ValueError(f"mode must be one of {sorted(modes)}")
+        _check(lib().sep_set_chase_rotations(self._h, modes[mode]))
 
     def get_timing(self) -> Tuple[float, float, int]:
         """(apply-kernel ms, preparation-kernels ms, apply launches) since the last reset."""

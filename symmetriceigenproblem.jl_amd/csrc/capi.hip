@@ -493,6 +493,21 @@ int sep_set_batched_config(sep_handle* h, int32_t fused_depth, int32_t rows_per_
     return set_batched_config(h, fused_depth, rows_per_lane);
 }
 
+int sep_set_chase_rotations(sep_handle* h, int32_t mode)
+{
+    if (!h) {
+        set_error("sep_set_chase_rotations: null handle");
+        return SEP_ERR_ARG;
+    }
+    if (mode != SEP_CHASE_EXACT && mode != SEP_CHASE_FAST) {
+        set_error("sep_set_chase_rotations: mode %d is neither SEP_CHASE_EXACT nor "
+                  "SEP_CHASE_FAST", (int)mode);
+        return SEP_ERR_ARG;
+    }
+    h->chase_fast = mode == SEP_CHASE_FAST;
+    return SEP_OK;
+}
+
 int sep_set_odd_order(sep_handle* h, int32_t allow)
 {
     if (!h) {

@@ -22,7 +22,7 @@ __global__ void __launch_bounds__(256) k_fill_identity(T* __restrict__ grid, int
     }
 }
 
-template <typename T>
+template <typename T, bool kFast>
 __global__ void __launch_bounds__(64) k_chase_block(T* dv, T* ev, int* lt, T* grid, int n,
                                                     int max_layers, T tol, int64_t maxiter,
                                                     int64_t* st)
@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(64) k_chase_block(T* dv, T* ev, int* lt, T* gr
     // previous block running beside it
     __builtin_amdgcn_s_setprio(3);
     chase::Problem<T> P{dv, ev, lt, grid, n, max_layers, (int64_t)(n - 1)};
-    chase::chase_block(P, tol, maxiter, st, (int)threadIdx.x);
+    chase::chase_block<T, true, kFast>(P, tol, maxiter, st, (int)threadIdx.x);
 }
 
 template <typename T>
@@ -110,8 +110,13 @@ int qr_algorithm_dev(sep_handle* h, int64_t n64, T* dv, T* ev, T* Q, int64_t ldq
             if (hipStreamWaitEvent(cs, applied[cur], 0) != hipSuccess) { rc = SEP_ERR_HIP; break; }
         }
         if ((rc = fill_identity<T>(cs, grid, pairs))) break;
-        hipLaunchKernelGGL((k_chase_block<T>), dim3(1), dim3(64), 0, cs, dv, ev,
-                           h->lt.as<int>(), grid, n, (int)max_layers, tol, maxiter, dst);
+        // the fast rotation is a Float64 option (Float32 chases always take the exact one)
+        if (std::is_same<T, double>::value && h->chase_fast)
+            hipLaunchKernelGGL((k_chase_block<T, true>), dim3(1), dim3(64), 0, cs, dv, ev,
+                               h->lt.as<int>(), grid, n, (int)max_layers, tol, maxiter, dst);
+        else
+            hipLaunchKernelGGL((k_chase_block<T, false>), dim3(1), dim3(64), 0, cs, dv, ev,
+                               h->lt.as<int>(), grid, n, (int)max_layers, tol, maxiter, dst);
         if (hipGetLastError() != hipSuccess) {
             set_error("chase launch failed");
             rc = SEP_ERR_HIP;

@@ -320,6 +320,40 @@ __device__ __forceinline__ Rot<double> givens_deferred(double f, double g, uint3
     return rot_common(f, g, sm);
 }
 
+// Opt-in fast rotation (Float64 single solves, sep_set_chase_rotations): c = f y, s = g y,
+// r = t y with t = f^2 + g^2 (one fma) and y = 1/sqrt(t) from v_rsq_f64 (relative error
+// below 2^-24) refined by one third-order step, y (1 + e/2 + 3 e^2 / 8) with e = 1 - t y^2
+// (truncation error ~ e^3 / 3 < 2^-70).  c, s and r are then within about an ulp of the
+// correctly rounded quotients f / r, g / r and of sqrt(t), but not always equal to them,
+// and r is never negated (the reference's sign rule for |f| > |g|, f < 0 is not applied:
+// G and -G give the same similarity), so the chase agrees with the reference to rounding
+// instead of bit for bit, in 13 vector instructions instead of 31.  The range test is
+// deferred as in givens_deferred: every step of the formula is accurate for t in
+// (2^-1000, 2^1000), and a chunk with a step outside it is run again with givens().
+constexpr uint32_t kFastLo = 0x01700000u;  // hi(2^-1000)
+constexpr uint32_t kFastHi = 0x7E700000u;  // hi(2^1000)
+constexpr uint32_t kRareFast = kFastHi - (kFastLo + 1);
+struct FastRotation {};
+__device__ __forceinline__ Rot<double> givens_fast(double f, double g, uint32_t& acc)
+{
+    const double t = __builtin_fma(f, f, g * g);
+    // (hi(t) - hi(2^-1000) - 1) mod 2^32 (t >= 0, so no sign bit to drop)
+    acc = max(acc, hi_word(t) - (kFastLo + 1u));
+    const double y0 = __builtin_amdgcn_rsq(t);
+    const double e = __builtin_fma(-(t * y0), y0, 1.0);
+    const double y = __builtin_fma(y0, __builtin_fma(e, 0.375, 0.5) * e, y0);
+    Rot<double> o;
+    o.c = f * y;
+    o.s = g * y;
+    o.r = t * y;
+    return o;
+}
+__device__ __forceinline__ Rot<double> step_rotation(FastRotation, double f, double g,
+                                                     uint32_t& acc)
+{
+    return givens_fast(f, g, acc);
+}
+
 // The chase step's rotation: with the range test deferred (Float64 chunks) or complete.
 __device__ __forceinline__ Rot<double> step_rotation(std::true_type, double f, double g,
                                                      uint32_t& acc)
@@ -403,17 +437,25 @@ __device__ __forceinline__ int64_t grid_pair(const Problem<T>& P, int i, int lay
     return (int64_t)(i - 1) + P.ldg * (layer - 1);
 }
 
-// 64-entry groups whose loads are in flight together: the deflation scan usually stops
-// within its first groups, the thickness maximum always reads its whole range
-constexpr int kScanBatch = 2;
+// 64-entry groups whose loads are in flight together in the deflation scan and the
+// thickness maximum.  The scan runs down from `to` until it finds a negligible ev, which
+// in a single large problem is usually far away (often the whole active range), and the
+// maximum always reads its whole range, so each is a chain of dependent memory round trips:
+// a single solve takes wide rounds (a sweep of an n = 2000 problem spends ~4 round trips
+// on them instead of ~20); the batched chase, whose problems are at most a few groups
+// long and whose waves share their SIMDs, keeps narrow ones.
+template <bool kSolo>
+struct ScanBatch {
+    static constexpr int scan = kSolo ? 8 : 2;
+    static constexpr int max = kSolo ? 32 : 8;
+};
 constexpr int kStepUnroll = 8;  // chase steps per iteration of the chunk loop
 // static LDS of the shared-SIMD chase (single_shift's chunk inputs), per wave
 constexpr unsigned kChaseLdsBytes = 2 * kWave * sizeof(double);
-constexpr int kMaxBatch = 8;
 
 // The highest 1-based q < to with is_small(q), scanning down to `start`, or start-1.
 // Each round issues the loads of kScanBatch groups of 64 indices before testing any.
-template <typename T>
+template <typename T, int kScanBatch>
 __device__ int deflation_scan(const Problem<T>& P, int to, int start, T tol, int lane)
 {
     for (int top = to - 1; top >= start; top -= kWave * kScanBatch) {
@@ -441,7 +483,7 @@ __device__ __forceinline__ int wave_max(int v)
 }
 
 // max(layer_thickness[from .. to]) (1-based, inclusive), kMaxBatch groups per round trip.
-template <typename T>
+template <typename T, int kMaxBatch>
 __device__ int max_thickness(const Problem<T>& P, int from, int to, int lane)
 {
     int v = 0;
@@ -469,14 +511,18 @@ __device__ int max_thickness(const Problem<T>& P, int from, int to, int lane)
 // instead of four v_readlane).  Measured per step, solo: 126 ns against 132 ns with the LDS
 // reads; batched, 4096 problems: 72-75 ms against 74-81 ms with v_readlane, and 80-81.5 ms
 // against 85 ms with the deferred test.
-template <typename T, bool kSolo = true>
+template <typename T, bool kSolo = true, bool kFast = false>
 __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int lane)
 {
     __shared__ T chunk_in[2][kWave];  // dv[i+1], ev[i+1] of the chunk's steps (!kSolo)
-    // kSolo: step j's (dv[i+1], ev[i+1]) at solo_in[j] (two pad entries for the read-ahead)
-    // and its outputs (nrm, B11'', c, s) at solo_out[j]
-    __shared__ T solo_in[kWave + 2][2];
-    __shared__ T solo_out[kWave][4];
+    // kSolo: chunks of R 64 steps (lane j handles the memory traffic of steps j, j + 64, ...);
+    // step j's (dv[i+1], ev[i+1]) at solo_in[j] (two pad entries for the read-ahead) and its
+    // outputs (nrm, B11'', c, s) at solo_out[j].  The batched chase keeps 64-step chunks
+    // (its outputs are deposited in registers, one step per lane).
+    constexpr int R = kSolo ? 2 : 1;
+    constexpr int kChunk = R * kWave;
+    __shared__ T solo_in[kChunk + 2][2];
+    __shared__ T solo_out[kChunk][4];
     T* __restrict__ dv = P.dv;
     T* __restrict__ ev = P.ev;
     int* __restrict__ lt = P.lt;
@@ -513,62 +559,79 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
     T p1 = H12pp;
     T d_cur = H22pp;
 
-    // ---- bulge chase, :91-140, in chunks of 64 steps
+    // ---- bulge chase, :91-140, in chunks of kChunk steps
     const int i_first = from + 1, i_last = to - 1;
     if (i_first <= i_last) {
-        // prefetch chunk 0: lane j <-> step i = i_first + j
-        T nD = T(0), nE = T(0);
-        int nL = 0;
-        {
-            const int i = i_first + lane;
-            if (i <= i_last) {
-                nD = dv[i];                      // dv[i+1] (1-based)
-                if (i + 2 <= to) nE = ev[i];     // ev[i+1] (1-based)
-                nL = lt[i - 1];
-            }
-        }
-        for (int i0u = i_first; i0u <= i_last; i0u += kWave) {
-            const int i0 = __builtin_amdgcn_readfirstlane(i0u);
-            const T D = nD, E = nE;
-            const int Lt = nL;
-            // this chunk's inputs are complete here (one wait per chunk, not per step)
-            asm volatile("" ::"v"(D), "v"(E));
-            // read back with one broadcast LDS read per value (a wave's LDS operations
-            // complete in order, so the previous chunk's reads are done)
-            if constexpr (!kSolo) {
-                chunk_in[0][lane] = D;
-                chunk_in[1][lane] = E;
-            } else {
-                solo_in[lane][0] = D;
-                solo_in[lane][1] = E;
-            }
-            {  // prefetch next chunk
-                const int i = i0 + kWave + lane;
-                nD = T(0);
-                nE = T(0);
-                nL = 0;
-                if (i <= i_last) {
-                    nD = dv[i];
-                    if (i + 2 <= to) nE = ev[i];
-                    nL = lt[i - 1];
+        // A chunk's inputs are staged in LDS, from which the steps read them back with one
+        // broadcast read per value.  They are staged at the end of the previous chunk,
+        // before its stores: the wait for the prefetch loads then covers only loads issued
+        // a chunk earlier, never the stores just issued (a wave's vector memory operations
+        // complete in order, so a wait for a load issued after a store also waits for the
+        // store).  A wave's LDS operations complete in order, so the chunk's own reads of
+        // the staging area are done when it is overwritten.
+        // lane j, slot r <-> step i = i0 + 64 r + j: dv[i+1], ev[i+1] and
+        // layer_thickness[i] (1-based)
+        T nD[R], nE[R];
+        int nL[R];
+        auto stage = [&]() {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if constexpr (!kSolo) {
+                    chunk_in[0][lane] = nD[r];
+                    chunk_in[1][lane] = nE[r];
+                } else {
+                    solo_in[kWave * r + lane][0] = nD[r];
+                    solo_in[kWave * r + lane][1] = nE[r];
                 }
             }
-            const int cnt = __builtin_amdgcn_readfirstlane(min(kWave, i_last - i0 + 1));
-            // outputs of step i0 + j are kept in lane j and stored once per chunk
-            T oR = T(0), oD = T(0), oC = T(0), oS = T(0);
+        };
+        auto prefetch = [&](int i0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = i0 + kWave * r + lane;
+                nD[r] = T(0);
+                nE[r] = T(0);
+                nL[r] = 0;
+                if (i <= i_last) {
+                    nD[r] = dv[i];
+                    if (i + 2 <= to) nE[r] = ev[i];
+                    nL[r] = lt[i - 1];
+                }
+            }
+        };
+        // kSolo: the inputs of a chunk's steps 0 and 1, read from the staging area at the end
+        // of the previous chunk, so that the chunk's first step does not wait for them
+        T l1D = T(0), l1E = T(0), l2D = T(0), l2E = T(0);
+        auto look = [&]() {
+            if constexpr (kSolo) {
+                l1D = solo_in[0][0];
+                l1E = solo_in[0][1];
+                l2D = solo_in[1][0];
+                l2E = solo_in[1][1];
+            }
+        };
+        prefetch(i_first);
+        stage();
+        look();
+        int Lt[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) Lt[r] = nL[r];
+        for (int i0u = i_first; i0u <= i_last; i0u += kChunk) {
+            const int i0 = __builtin_amdgcn_readfirstlane(i0u);
+            prefetch(i0 + kChunk);  // next chunk
+            const int cnt = __builtin_amdgcn_readfirstlane(min(kChunk, i_last - i0 + 1));
+            // outputs of step i0 + 64 r + j are kept in lane j, slot r and stored once per
+            // chunk
+            T oR[R], oD[R], oC[R], oS[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) oR[r] = oD[r] = oC[r] = oS[r] = T(0);
             // the chunk's entry state, for the rerun of a chunk that left the common range
             const T p1_0 = p1, d_0 = d_cur, e_0 = e_cur, b_0 = bulge;
             uint32_t acc = 0;
             auto run = [&](auto deferred) {
                 uint64_t jm = 1;  // lane mask of step j
                 // kSolo: the inputs of steps j + 1 and j + 2, read from LDS ahead of use
-                T a1D = T(0), a1E = T(0), a2D = T(0), a2E = T(0);
-                if constexpr (kSolo) {
-                    a1D = solo_in[0][0];
-                    a1E = solo_in[0][1];
-                    a2D = solo_in[1][0];
-                    a2E = solo_in[1][1];
-                }
+                T a1D = l1D, a1E = l1E, a2D = l2D, a2E = l2E;
                 auto step = [&](int j) {
                     T B22, B23n;
                     if constexpr (kSolo) {
@@ -598,7 +661,7 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                         solo_out[j][2] = c;
                         solo_out[j][3] = s;
                     } else {
-                        capture_lane(oR, oD, oC, oS, G.r, B11pp, c, s, jm);
+                        capture_lane(oR[0], oD[0], oC[0], oS[0], G.r, B11pp, c, s, jm);
                         jm <<= 1;
                     }
                     p1 = B12pp;
@@ -620,8 +683,12 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                 for (; j < cnt; ++j) step(j);
             };
             if constexpr (std::is_same<T, double>::value && kSolo) {
-                run(std::true_type{});
-                if (__builtin_expect(__builtin_amdgcn_readfirstlane((int)(acc >= kRareAcc)), 0)) {
+                if constexpr (kFast)
+                    run(FastRotation{});
+                else
+                    run(std::true_type{});
+                const uint32_t rare = kFast ? kRareFast : kRareAcc;
+                if (__builtin_expect(__builtin_amdgcn_readfirstlane((int)(acc >= rare)), 0)) {
                     p1 = p1_0;
                     d_cur = d_0;
                     e_cur = e_0;
@@ -632,20 +699,38 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                 run(std::false_type{});
             }
             if constexpr (kSolo) {
-                oR = solo_out[lane][0];
-                oD = solo_out[lane][1];
-                oC = solo_out[lane][2];
-                oS = solo_out[lane][3];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    oR[r] = solo_out[kWave * r + lane][0];
+                    oD[r] = solo_out[kWave * r + lane][1];
+                    oC[r] = solo_out[kWave * r + lane][2];
+                    oS[r] = solo_out[kWave * r + lane][3];
+                }
+            }
+            int layer[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) layer[r] = Lt[r] + 1;
+            stage();
+            look();
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                Lt[r] = nL[r];
+                // take the next chunk's thickness into its register here, before the
+                // stores, so that the wait for its load cannot land behind them
+                asm volatile("" : "+v"(Lt[r]));
             }
             // coalesced stores of the chunk; the next chunk was prefetched from entries
-            // these stores do not touch (dv/ev beyond i0 + 62, lt beyond i0 + 62)
-            if (lane < cnt) {
-                const int i = i0 + lane;
-                const int layer = Lt + 1;
-                ev[i - 2] = oR;   // ev[i-1] (1-based) = nrm
-                dv[i - 1] = oD;   // dv[i]   (1-based) = B11''
-                lt[i - 1] = layer;
-                store_pair(grid + 2 * grid_pair(P, i, layer), oC, oS);
+            // these stores do not touch (dv/ev beyond i0 + kChunk - 2, lt beyond
+            // i0 + kChunk - 2)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (kWave * r + lane < cnt) {
+                    const int i = i0 + kWave * r + lane;
+                    ev[i - 2] = oR[r];   // ev[i-1] (1-based) = nrm
+                    dv[i - 1] = oD[r];   // dv[i]   (1-based) = B11''
+                    lt[i - 1] = layer[r];
+                    store_pair(grid + 2 * grid_pair(P, i, layer[r]), oC[r], oS[r]);
+                }
             }
         }
     }
@@ -670,7 +755,7 @@ enum StateWord {
 
 // One outer iteration of qr_algorithm! without the apply (src/bulge_chasing.jl:166-276):
 // layer_thickness is reset here; the grid must hold identity rotations on entry.
-template <typename T, bool kSolo = true>
+template <typename T, bool kSolo = true, bool kFast = false>
 __device__ void chase_block(const Problem<T>& P, T tol, int64_t maxiter, int64_t* st, int lane)
 {
     const int start = 1;
@@ -689,7 +774,8 @@ __device__ void chase_block(const Problem<T>& P, T tol, int64_t maxiter, int64_t
         }
         int from = to;
         if (from > start)
-            from = __builtin_amdgcn_readfirstlane(deflation_scan(P, to, start, tol, lane) + 1);
+            from = __builtin_amdgcn_readfirstlane(
+                deflation_scan<T, ScanBatch<kSolo>::scan>(P, to, start, tol, lane) + 1);
         if (to <= start) break;
         if (from == to) {
             if (lane == 0) P.ev[from - 2] = T(0);
@@ -697,13 +783,14 @@ __device__ void chase_block(const Problem<T>& P, T tol, int64_t maxiter, int64_t
             if (to == lbea) lbea -= 1;
             to -= 1;
         } else {
-            const int thick = __builtin_amdgcn_readfirstlane(max_thickness(P, from, to, lane));
+            const int thick = __builtin_amdgcn_readfirstlane(
+                max_thickness<T, ScanBatch<kSolo>::max>(P, from, to, lane));
             if (thick == P.max_layers) {
                 to = from - 1;
             } else {
                 const T lam = wilkinson(P.dv[to - 2], P.ev[to - 2], P.dv[to - 1]);
                 did = 1;
-                single_shift<T, kSolo>(P, from, to, lam, lane);
+                single_shift<T, kSolo, kFast>(P, from, to, lam, lane);
                 // the next scan and sweep read what this sweep's lanes stored
                 __threadfence_block();
                 nrot += to - from;
@@ -715,7 +802,7 @@ __device__ void chase_block(const Problem<T>& P, T tol, int64_t maxiter, int64_t
         if (to <= start) break;
     }
     int maxlayer = 0;
-    if (did) maxlayer = max_thickness(P, 1, P.n, lane);
+    if (did) maxlayer = max_thickness<T, ScanBatch<kSolo>::max>(P, 1, P.n, lane);
     if (lane == 0) {
         st[ST_TO] = lbea;
         st[ST_LBEA] = lbea;

@@ -117,6 +117,8 @@ struct sep_handle {
     int force_kf = 0, force_r = 0;  // sep_set_batched_config (0: autotuned / default)
     int apply_geom = 0;  // apply geometry forced by SEP_APPLY (0: chosen by size)
     bool allow_odd = false;  // sep_set_odd_order: solves accept odd n
+    bool chase_fast = false;  // sep_set_chase_rotations: Float64 solves chase with the
+                              // fast (rounding-level) rotation
     // kernel timing (sep_set_timing): event pairs recorded around apply / prep kernels
     bool timing = false;
     std::vector<hipEvent_t> tev;            // pool, reused
