@@ -22,7 +22,15 @@ __global__ void __launch_bounds__(256) k_fill_identity(T* __restrict__ grid, int
     }
 }
 
-template <typename T, bool kFast>
+// Problems up to this order keep dv, ev and layer_thickness in LDS for the whole block
+// (20 bytes per index in Float64: 120 KiB beside the chase's 6 KiB of chunk staging).
+// The chase's own memory traffic is a chain of dependent round trips -- the deflation scan
+// from `to` downwards, the thickness maximum, the shift's and the first rotation's
+// operands, each chunk's inputs -- which take ~0.1 us each from LDS against ~0.7 us from
+// L2; only the (c, s) grid goes to global memory.  Larger problems chase in global memory.
+constexpr int kChaseLdsN = 6144;
+
+template <typename T, bool kFast, bool kLds>
 __global__ void __launch_bounds__(64) k_chase_block(T* dv, T* ev, int* lt, T* grid, int n,
                                                     int max_layers, T tol, int64_t maxiter,
                                                     int64_t* st)
@@ -30,8 +38,48 @@ __global__ void __launch_bounds__(64) k_chase_block(T* dv, T* ev, int* lt, T* gr
     // the chase is the solve's critical path: its wave issues ahead of the apply of the
     // previous block running beside it
     __builtin_amdgcn_s_setprio(3);
-    chase::Problem<T> P{dv, ev, lt, grid, n, max_layers, (int64_t)(n - 1)};
-    chase::chase_block<T, true, kFast>(P, tol, maxiter, st, (int)threadIdx.x);
+    const int lane = (int)threadIdx.x;
+    if constexpr (kLds) {
+        // dynamic LDS (chase_lds_bytes(n)): it lies after the chase's static chunk staging,
+        // whose per-step accesses then keep small immediate offsets
+        extern __shared__ __align__(16) unsigned char chase_lds[];
+        T* s_dv = reinterpret_cast<T*>(chase_lds);
+        T* s_ev = s_dv + n;
+        int* s_lt = reinterpret_cast<int*>(s_ev + n);
+        for (int q = lane; q < n; q += 64) s_dv[q] = dv[q];
+        for (int q = lane; q < n - 1; q += 64) s_ev[q] = ev[q];
+        __syncthreads();
+        // layer_thickness lives only within a block (chase_block resets it), so the LDS
+        // copy needs neither loading nor writing back
+        chase::Problem<T> P{s_dv, s_ev, s_lt, grid, n, max_layers, (int64_t)(n - 1)};
+        chase::chase_block<T, true, kFast>(P, tol, maxiter, st, lane);
+        __syncthreads();
+        for (int q = lane; q < n; q += 64) dv[q] = s_dv[q];
+        for (int q = lane; q < n - 1; q += 64) ev[q] = s_ev[q];
+    } else {
+        chase::Problem<T> P{dv, ev, lt, grid, n, max_layers, (int64_t)(n - 1)};
+        chase::chase_block<T, true, kFast>(P, tol, maxiter, st, lane);
+    }
+}
+
+// dv, ev (n entries each) and layer_thickness of the LDS-resident chase
+template <typename T>
+constexpr size_t chase_lds_bytes(int n)
+{
+    return (2 * sizeof(T) + sizeof(int)) * (size_t)n;
+}
+
+template <typename T, bool kFast>
+void launch_chase(hipStream_t s, T* dv, T* ev, int* lt, T* grid, int n, int max_layers, T tol,
+                  int64_t maxiter, int64_t* st)
+{
+    if (n <= kChaseLdsN)
+        hipLaunchKernelGGL((k_chase_block<T, kFast, true>), dim3(1), dim3(64),
+                           (unsigned)chase_lds_bytes<T>(n), s, dv, ev, lt, grid, n, max_layers,
+                           tol, maxiter, st);
+    else
+        hipLaunchKernelGGL((k_chase_block<T, kFast, false>), dim3(1), dim3(64), 0, s, dv, ev, lt,
+                           grid, n, max_layers, tol, maxiter, st);
 }
 
 template <typename T>
@@ -112,11 +160,11 @@ int qr_algorithm_dev(sep_handle* h, int64_t n64, T* dv, T* ev, T* Q, int64_t ldq
         if ((rc = fill_identity<T>(cs, grid, pairs))) break;
         // the fast rotation is a Float64 option (Float32 chases always take the exact one)
         if (std::is_same<T, double>::value && h->chase_fast)
-            hipLaunchKernelGGL((k_chase_block<T, true>), dim3(1), dim3(64), 0, cs, dv, ev,
-                               h->lt.as<int>(), grid, n, (int)max_layers, tol, maxiter, dst);
+            launch_chase<T, true>(cs, dv, ev, h->lt.as<int>(), grid, n, (int)max_layers, tol,
+                                  maxiter, dst);
         else
-            hipLaunchKernelGGL((k_chase_block<T, false>), dim3(1), dim3(64), 0, cs, dv, ev,
-                               h->lt.as<int>(), grid, n, (int)max_layers, tol, maxiter, dst);
+            launch_chase<T, false>(cs, dv, ev, h->lt.as<int>(), grid, n, (int)max_layers, tol,
+                                   maxiter, dst);
         if (hipGetLastError() != hipSuccess) {
             set_error("chase launch failed");
             rc = SEP_ERR_HIP;
