@@ -338,6 +338,20 @@ def batched_section(sep, h, torch, np, ex):
         t = time.perf_counter()
         conv, iters = sep.qr_batched_(ns, off_d, off_q, dvb, evb, Qb, handle=h)
         best = min(best, time.perf_counter() - t)
+    # the same batch with the opt-in fast chase rotation (results equal to rounding)
+    best_fast = float("inf")
+    h.set_chase_rotations("fast")
+    try:
+        for _ in range(2):
+            dvb, evb, Qb = fresh()
+            t = time.perf_counter()
+            conv_f, _ = sep.qr_batched_(ns, off_d, off_q, dvb, evb, Qb, handle=h)
+            best_fast = min(best_fast, time.perf_counter() - t)
+    finally:
+        h.set_chase_rotations("exact")
+    ex["batched_fast_rotations_s"] = round(best_fast, 4)
+    ex["batched_fast_rotations_all_converged"] = bool(conv_f.all())
+    log(f"[bench] batched B={B}, fast rotations: {best_fast * 1e3:.1f} ms")
     # the plain per-problem GPU loop (same library, one qr_algorithm! per problem) on every
     # 64th problem, extrapolated to the batch by problem count; inputs are staged on the
     # device before the clock starts, as for the batched solve

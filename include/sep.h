@@ -152,12 +152,13 @@ int sep_set_batched_config(sep_handle *h, int32_t fused_depth, int32_t rows_per_
  * default keeps the reference's behaviour (SEP_ERR_ODD_N). */
 int sep_set_odd_order(sep_handle *h, int32_t allow);
 
-/* Rotation arithmetic of the Float64 solves on this handle (sep_qr_algorithm_f64*):
+/* Rotation arithmetic of the Float64 solves on this handle (sep_qr_algorithm_f64* and
+ * sep_qr_batched_f64_dev):
  * SEP_CHASE_EXACT (the default) computes every chase rotation as LinearAlgebra's
  * givensAlgorithm does (src/simple_givens.jl:20-23, src/bulge_chasing.jl:97), so D, E, the
  * (c,s) grid and Q equal the reference's bit for bit; SEP_CHASE_FAST computes the bulge
  * rotations from one reciprocal square root (c = f/r, s = g/r, r = sqrt(f^2+g^2) each
- * within about an ulp, r >= 0), a shorter dependency chain per chase step, so results
+ * within about an ulp, c >= 0), a shorter dependency chain per chase step, so results
  * agree with the reference to rounding (eigenvalues, residual and orthogonality at the
  * same tolerances) but not bitwise.  No counterpart in the reference.  Float32 solves
  * always use the exact rotation. */

@@ -194,10 +194,10 @@ class Handle:
         _check(lib().sep_set_odd_order(self._h, int(bool(allow))))
 
     def set_chase_rotations(self, mode: str = "exact") -> None:
-        """Rotation arithmetic of this handle's Float64 solves: "exact" (the default, the
-        reference's givensAlgorithm, bitwise-equal results) or "fast" (one reciprocal
-        square root per chase rotation, results equal to rounding; no counterpart in the
-        reference).  Float32 solves always use the exact rotation."""
+        """Rotation arithmetic of this handle's Float64 solves, single and batched:
+        "exact" (the default, the reference's givensAlgorithm, bitwise-equal results) or
+        "fast" (one reciprocal square root per chase rotation, results equal to rounding;
+        no counterpart in the reference).  Float32 solves always use the exact rotation."""
         modes = {"exact": 0, "fast": 1}
         if mode not in modes:
             raise ValueError(f"mode must be one of {sorted(modes)}")

@@ -77,6 +77,7 @@ constexpr int kLdsChaseMaxN = 3072;
 // on a global memory that the concurrent apply keeps busy.  The block's results are
 // copied back (dv, ev and the layer counts the apply reads); the grid goes straight to
 // global memory.
+template <bool kFast>
 __global__ void __launch_bounds__(kLanes)
     k_chase_all(const BProb* __restrict__ probs, const int* __restrict__ order, double* dv,
                 double* ev, int* lt, double* grid, int4* info, int64_t* st_all, int ML, int nmax,
@@ -116,7 +117,7 @@ __global__ void __launch_bounds__(kLanes)
         }
         __syncthreads();
         chase::Problem<double> PL{ld, le, ll, P.grid, pr.n, ML, (int64_t)(pr.n - 1)};
-        chase::chase_block<double, kBatchedSolo>(PL, tol, 100 * (int64_t)pr.n, st, lane);
+        chase::chase_block<double, kBatchedSolo, kFast>(PL, tol, 100 * (int64_t)pr.n, st, lane);
         __syncthreads();
         for (int q = lane; q < pr.n; q += kLanes) {
             P.dv[q] = ld[q];
@@ -124,7 +125,7 @@ __global__ void __launch_bounds__(kLanes)
             P.lt[q] = ll[q];
         }
     } else {
-        chase::chase_block<double, kBatchedSolo>(P, tol, 100 * (int64_t)pr.n, st, lane);
+        chase::chase_block<double, kBatchedSolo, kFast>(P, tol, 100 * (int64_t)pr.n, st, lane);
     }
     __threadfence_block();
     if (lane == 0) {
@@ -519,9 +520,14 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
         const int b = (int)(r & 1);
         hipError_t e = hipSuccess;
         if (!first && (e = hipStreamWaitEvent(cs, applied[b], 0)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_chase_all, dim3((unsigned)B), dim3(kLanes), reserve, cs, d_probs,
-                           d_order, dv, ev, d_lt[b], d_grid[b], d_depth[b], d_st, cfg.ML, L.nmax,
-                           d_count, in_lds);
+        if (h->chase_fast)
+            hipLaunchKernelGGL(k_chase_all<true>, dim3((unsigned)B), dim3(kLanes), reserve, cs,
+                               d_probs, d_order, dv, ev, d_lt[b], d_grid[b], d_depth[b], d_st,
+                               cfg.ML, L.nmax, d_count, in_lds);
+        else
+            hipLaunchKernelGGL(k_chase_all<false>, dim3((unsigned)B), dim3(kLanes), reserve, cs,
+                               d_probs, d_order, dv, ev, d_lt[b], d_grid[b], d_depth[b], d_st,
+                               cfg.ML, L.nmax, d_count, in_lds);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = hipMemcpyAsync(pin, d_count, sizeof(int), hipMemcpyDeviceToHost, cs)) !=
                 hipSuccess ||
@@ -567,7 +573,8 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
             (int64_t)d_st, (int64_t)d_depth[0], (int64_t)d_lt[0], (int64_t)d_grid[0],
             (int64_t)d_order,
             (int64_t)d_lt[1], (int64_t)d_grid[1], (int64_t)d_count, (int64_t)pin, B, nwork,
-            L.nmax, cfg.ML, cfg.KF, cfg.R, cfg.CW, cfg.AW, (int64_t)cs, (int64_t)as};
+            L.nmax, cfg.ML, cfg.KF, cfg.R, cfg.CW, cfg.AW, (int64_t)cs, (int64_t)as,
+            (int64_t)h->chase_fast};  // the chase kernel the rounds launch
         if (!h->batch_graph || h->batch_graph_key != key) {
             if (h->batch_graph) {
                 (void)hipGraphExecDestroy(h->batch_graph);

@@ -323,13 +323,15 @@ __device__ __forceinline__ Rot<double> givens_deferred(double f, double g, uint3
 // Opt-in fast rotation (Float64 single solves, sep_set_chase_rotations): c = f y, s = g y,
 // r = t y with t = f^2 + g^2 (one fma) and y = 1/sqrt(t) from v_rsq_f64 (relative error
 // below 2^-24) refined by one third-order step, y (1 + e/2 + 3 e^2 / 8) with e = 1 - t y^2
-// (truncation error ~ e^3 / 3 < 2^-70).  c, s and r are then within about an ulp of the
-// correctly rounded quotients f / r, g / r and of sqrt(t), but not always equal to them,
-// and r is never negated (the reference's sign rule for |f| > |g|, f < 0 is not applied:
-// G and -G give the same similarity), so the chase agrees with the reference to rounding
-// instead of bit for bit, in 13 vector instructions instead of 31.  The range test is
-// deferred as in givens_deferred: every step of the formula is accurate for t in
-// (2^-1000, 2^1000), and a chunk with a step outside it is run again with givens().
+// (truncation error ~ e^3 / 3 < 2^-70).  y takes the sign of f, so c = |f| y >= 0 and r
+// carries f's sign: where the reference negates (|f| > |g|, f < 0) this agrees with it,
+// elsewhere it may differ by the sign of G (G and -G give the same similarity), and c >= 0
+// keeps the appliers' shear forms, with coefficients s / (1 + c), well conditioned.  c, s
+// and r are within about an ulp of the reference's correctly rounded quotients, but not
+// always equal to them, so the chase agrees with the reference to rounding instead of bit
+// for bit, in 14 vector instructions instead of 31.  The range test is deferred as in
+// givens_deferred: every step of the formula is accurate for t in (2^-1000, 2^1000), and a
+// chunk with a step outside it is run again with givens().
 constexpr uint32_t kFastLo = 0x01700000u;  // hi(2^-1000)
 constexpr uint32_t kFastHi = 0x7E700000u;  // hi(2^1000)
 constexpr uint32_t kRareFast = kFastHi - (kFastLo + 1);
@@ -341,7 +343,7 @@ __device__ __forceinline__ Rot<double> givens_fast(double f, double g, uint32_t&
     acc = max(acc, hi_word(t) - (kFastLo + 1u));
     const double y0 = __builtin_amdgcn_rsq(t);
     const double e = __builtin_fma(-(t * y0), y0, 1.0);
-    const double y = __builtin_fma(y0, __builtin_fma(e, 0.375, 0.5) * e, y0);
+    const double y = __builtin_copysign(__builtin_fma(y0, __builtin_fma(e, 0.375, 0.5) * e, y0), f);
     Rot<double> o;
     o.c = f * y;
     o.s = g * y;
@@ -390,32 +392,6 @@ __device__ __forceinline__ T wilkinson(T H11, T H12, T H22)
     return lam;
 }
 
-// Lane j of (oR, oD, oC, oS) <- (r, d, c, s) (values the same in every lane): one masked
-// block of 64-bit moves with EXEC = exec & lane mask `m`, restored inside the block.
-__device__ __forceinline__ void capture_lane(double& oR, double& oD, double& oC, double& oS,
-                                             double r, double d, double c, double s,
-                                             uint64_t m)
-{
-    uint64_t save;
-    asm volatile("s_and_saveexec_b64 %4, %5\n"
-                 "v_mov_b64 %0, %6\n"
-                 "v_mov_b64 %1, %7\n"
-                 "v_mov_b64 %2, %8\n"
-                 "v_mov_b64 %3, %9\n"
-                 "s_mov_b64 exec, %4\n"
-                 : "+v"(oR), "+v"(oD), "+v"(oC), "+v"(oS), "=&s"(save)
-                 : "s"(m), "v"(r), "v"(d), "v"(c), "v"(s));
-}
-__device__ __forceinline__ void capture_lane(float& oR, float& oD, float& oC, float& oS, float r,
-                                             float d, float c, float s, uint64_t m)
-{
-    const bool mine = ((m >> (__lane_id() & 63)) & 1) != 0;
-    oR = mine ? r : oR;
-    oD = mine ? d : oD;
-    oC = mine ? c : oC;
-    oS = mine ? s : oS;
-}
-
 // Problem view.  dv (n), ev (n-1), lt (n) and the (c,s) grid: the reference's
 // (n-1) x max_layers matrix of interleaved pairs (leading dimension ldg pairs).  Indices
 // in the member functions are 1-based.
@@ -450,8 +426,8 @@ struct ScanBatch {
     static constexpr int max = kSolo ? 32 : 8;
 };
 constexpr int kStepUnroll = 8;  // chase steps per iteration of the chunk loop
-// static LDS of the shared-SIMD chase (single_shift's chunk inputs), per wave
-constexpr unsigned kChaseLdsBytes = 2 * kWave * sizeof(double);
+// static LDS of the batched chase (single_shift's chunk staging, 64-step chunks), per wave
+constexpr unsigned kChaseLdsBytes = ((kWave + 2) * 2 + kWave * 4) * sizeof(double);
 
 // The highest 1-based q < to with is_small(q), scanning down to `start`, or start-1.
 // Each round issues the loads of kScanBatch groups of 64 indices before testing any.
@@ -514,7 +490,6 @@ __device__ int max_thickness(const Problem<T>& P, int from, int to, int lane)
 template <typename T, bool kSolo = true, bool kFast = false>
 __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int lane)
 {
-    __shared__ T chunk_in[2][kWave];  // dv[i+1], ev[i+1] of the chunk's steps (!kSolo)
     // kSolo: chunks of R 64 steps (lane j handles the memory traffic of steps j, j + 64, ...);
     // step j's (dv[i+1], ev[i+1]) at solo_in[j] (two pad entries for the read-ahead) and its
     // outputs (nrm, B11'', c, s) at solo_out[j].  The batched chase keeps 64-step chunks
@@ -576,13 +551,8 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
         auto stage = [&]() {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                if constexpr (!kSolo) {
-                    chunk_in[0][lane] = nD[r];
-                    chunk_in[1][lane] = nE[r];
-                } else {
-                    solo_in[kWave * r + lane][0] = nD[r];
-                    solo_in[kWave * r + lane][1] = nE[r];
-                }
+                solo_in[kWave * r + lane][0] = nD[r];
+                solo_in[kWave * r + lane][1] = nE[r];
             }
         };
         auto prefetch = [&](int i0) {
@@ -599,16 +569,14 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                 }
             }
         };
-        // kSolo: the inputs of a chunk's steps 0 and 1, read from the staging area at the end
-        // of the previous chunk, so that the chunk's first step does not wait for them
+        // the inputs of a chunk's steps 0 and 1, read from the staging area at the end of
+        // the previous chunk, so that the chunk's first step does not wait for them
         T l1D = T(0), l1E = T(0), l2D = T(0), l2E = T(0);
         auto look = [&]() {
-            if constexpr (kSolo) {
-                l1D = solo_in[0][0];
-                l1E = solo_in[0][1];
-                l2D = solo_in[1][0];
-                l2E = solo_in[1][1];
-            }
+            l1D = solo_in[0][0];
+            l1E = solo_in[0][1];
+            l2D = solo_in[1][0];
+            l2E = solo_in[1][1];
         };
         prefetch(i_first);
         stage();
@@ -623,27 +591,18 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
             // outputs of step i0 + 64 r + j are kept in lane j, slot r and stored once per
             // chunk
             T oR[R], oD[R], oC[R], oS[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r) oR[r] = oD[r] = oC[r] = oS[r] = T(0);
             // the chunk's entry state, for the rerun of a chunk that left the common range
             const T p1_0 = p1, d_0 = d_cur, e_0 = e_cur, b_0 = bulge;
             uint32_t acc = 0;
             auto run = [&](auto deferred) {
-                uint64_t jm = 1;  // lane mask of step j
-                // kSolo: the inputs of steps j + 1 and j + 2, read from LDS ahead of use
+                // the inputs of steps j + 1 and j + 2, read from LDS ahead of use
                 T a1D = l1D, a1E = l1E, a2D = l2D, a2E = l2E;
                 auto step = [&](int j) {
-                    T B22, B23n;
-                    if constexpr (kSolo) {
-                        B22 = a1D;
-                        B23n = a1E;
-                        a1D = a2D;
-                        a1E = a2E;
-                        a2D = solo_in[j + 2][0];
-                        a2E = solo_in[j + 2][1];
-                    } else {
-                        B22 = chunk_in[0][j];
-                    }
+                    const T B22 = a1D, B23 = a1E;
+                    a1D = a2D;
+                    a1E = a2E;
+                    a2D = solo_in[j + 2][0];
+                    a2E = solo_in[j + 2][1];
                     const Rot<T> G = step_rotation(deferred, p1, bulge, acc);
                     const T c = G.c, s = G.s;
                     const T B11 = d_cur, B21 = e_cur;
@@ -654,21 +613,15 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                     const T B11pp = __builtin_fma(B11p, c, B12p * s);
                     const T B12pp = __builtin_fma(B11p, -s, B12p * c);
                     const T B22pp = __builtin_fma(B21p, -s, B22p * c);
-                    if constexpr (kSolo) {
-                        // every lane writes the same values (the LDS pipe, not the VALU)
-                        solo_out[j][0] = G.r;
-                        solo_out[j][1] = B11pp;
-                        solo_out[j][2] = c;
-                        solo_out[j][3] = s;
-                    } else {
-                        capture_lane(oR[0], oD[0], oC[0], oS[0], G.r, B11pp, c, s, jm);
-                        jm <<= 1;
-                    }
+                    // every lane writes the same values (the LDS pipe, not the VALU)
+                    solo_out[j][0] = G.r;
+                    solo_out[j][1] = B11pp;
+                    solo_out[j][2] = c;
+                    solo_out[j][3] = s;
                     p1 = B12pp;
                     d_cur = B22pp;
                     // B23 = ev[i+1]; the last step of the sweep has none: its prefetch slot
                     // holds 0, and the bulge / e_cur it then produces are never read
-                    const T B23 = kSolo ? B23n : chunk_in[1][j];
                     bulge = s * B23;
                     e_cur = c * B23;
                 };
@@ -682,7 +635,7 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                 }
                 for (; j < cnt; ++j) step(j);
             };
-            if constexpr (std::is_same<T, double>::value && kSolo) {
+            if constexpr (std::is_same<T, double>::value && (kSolo || kFast)) {
                 if constexpr (kFast)
                     run(FastRotation{});
                 else
@@ -698,14 +651,12 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
             } else {
                 run(std::false_type{});
             }
-            if constexpr (kSolo) {
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    oR[r] = solo_out[kWave * r + lane][0];
-                    oD[r] = solo_out[kWave * r + lane][1];
-                    oC[r] = solo_out[kWave * r + lane][2];
-                    oS[r] = solo_out[kWave * r + lane][3];
-                }
+            for (int r = 0; r < R; ++r) {
+                oR[r] = solo_out[kWave * r + lane][0];
+                oD[r] = solo_out[kWave * r + lane][1];
+                oC[r] = solo_out[kWave * r + lane][2];
+                oS[r] = solo_out[kWave * r + lane][3];
             }
             int layer[R];
 #pragma unroll
