@@ -8,21 +8,24 @@
 // The sweep is a serial recurrence; the wavefront's 64 lanes are used around it:
 //  * the deflation scan and the capacity reduction test 64 indices per step (ballot /
 //    lane max);
-//  * a sweep is processed in chunks of 64 steps: lane j preloads dv[i+1], ev[i+1] and
-//    layer_thickness[i] of step i = i0 + j (values no earlier step of the sweep
-//    changes) for the next chunk while the current one runs, the recurrence reads them
-//    with v_readlane and deposits its finished outputs (ev[i-1], dv[i], c, s) back in
-//    lane j with a lane select, and the chunk is stored with one coalesced store per
-//    array.  The recurrence itself therefore never waits on memory;
+//  * a sweep is processed in chunks of 128 steps (64 in the batched chase): lane j preloads
+//    dv[i+1], ev[i+1] and layer_thickness[i] of steps i = i0 + j, i0 + 64 + j (values no
+//    earlier step of the sweep changes) for the next chunk while the current one runs and
+//    stages them in LDS; the recurrence reads them from there two steps ahead and writes
+//    its finished outputs (ev[i-1], dv[i], c, s) to LDS, both on the LDS pipe instead of
+//    the vector ALU, and the chunk is stored with one coalesced store per array.  The
+//    recurrence itself therefore never waits on memory;
 //  * in Float64 the rotation's range test (extreme magnitudes take the complete DLARTG)
 //    can be deferred to the end of a chunk: each step folds its operands' exponent words
 //    into a running maximum, and a chunk that left the common range is run again from its
 //    entry state with the per-step test (single solves defer; the batched chase, where
-//    vector issue is shared by several waves, keeps the per-step scalar test and reads
-//    the chunk inputs from LDS instead of with v_readlane).
+//    vector issue is shared by several waves, keeps the per-step scalar test);
+//  * single solves up to a few thousand indices run with dv, ev and layer_thickness in
+//    LDS (the kernel passes LDS pointers in the Problem view).
 // Arithmetic follows the reference operation by operation without contraction, and
 // uses IEEE division and square root, so the grid, dv, ev and iteration count equal
-// the C restatement's bit for bit.
+// the C restatement's bit for bit -- except under the opt-in fast rotation (kFast, see
+// givens_fast), whose results agree with the reference to rounding.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -47,19 +50,6 @@ struct Consts<float> {
     static constexpr float safmn2 = 0x1p-51f;
     static constexpr float safmx2 = 0x1p51f;
 };
-
-__device__ __forceinline__ double rdlane(double v, int j)
-{
-    const uint64_t u = __double_as_longlong(v);
-    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)u, j);
-    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), j);
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
-__device__ __forceinline__ float rdlane(float v, int j)
-{
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
-}
-__device__ __forceinline__ int rdlane(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
 
 __device__ __forceinline__ double tabs(double x) { return __builtin_fabs(x); }
 
@@ -478,15 +468,14 @@ __device__ int max_thickness(const Problem<T>& P, int from, int to, int lane)
 
 // single_shift!(H, from, to, mu, givens, layer_thickness), src/bulge_chasing.jl:11-144,
 // executed by the whole wave (all lanes compute the uniform recurrence; lane j owns the
-// memory traffic of step i0 + j in each 64-step chunk).
+// memory traffic of steps i0 + j, i0 + 64 + j, ... of each chunk).
 // kSolo: the chase wave has its SIMD to itself (single solves), so the step is bound by
-// the wave's own latency: the rotation's range test is deferred to the end of the chunk and
-// the chunk inputs are read with v_readlane.  Otherwise (the batched chase, several waves
-// and the apply per SIMD) vector issue is the shared resource: the range test stays on the
-// scalar unit per step and the inputs are read back from an LDS copy (two broadcast reads
-// instead of four v_readlane).  Measured per step, solo: 126 ns against 132 ns with the LDS
-// reads; batched, 4096 problems: 72-75 ms against 74-81 ms with v_readlane, and 80-81.5 ms
-// against 85 ms with the deferred test.
+// the wave's own issue: the rotation's range test is deferred to the end of the chunk, and
+// chunks are 128 steps long.  Otherwise (the batched chase, several waves and the apply
+// per SIMD) the range test stays on the scalar unit per step (80-81.5 ms against 85 ms per
+// 4096-problem batch with the deferred test) and chunks are 64 steps, which keeps the
+// chunk staging at 3 KB of LDS per wave.  kFast (Float64): the fast rotation and the
+// dsteqr-style 2x2 update, with the deferred range test in both forms.
 template <typename T, bool kSolo = true, bool kFast = false>
 __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int lane)
 {
