@@ -64,32 +64,28 @@ namespace {
 
 constexpr int kLanes = 64;
 
-// flag = 1 if the grid needs the five-instruction path: some |c^2 + s^2 - 1| above 1/64,
-// or a c or s that is not finite.  Every other grid runs the scaled three-shear form (see
-// k_colscale), which applies the given (c, s) to rounding: normalised pairs, and pairs
-// that are only near normalised (c^2 + s^2 = 1 + delta), alike.  The residual is
-// evaluated as fma(c, c, fma(s, s, -1)), accurate to a fraction of eps.
+// Normalisation flag of a pass: 1 if the grid needs the five-instruction path (some
+// |c^2 + s^2 - 1| above 1/64 among the pass's pairs, or a c or s that is not finite); every
+// other grid runs the scaled three-shear form (see k_colscale), which applies the given
+// (c, s) to rounding: normalised pairs, and pairs that are only near normalised
+// (c^2 + s^2 = 1 + delta), alike.  k_colscale evaluates it on the pairs it loads (the residual
+// as fma(c, c, fma(s, s, -1)), accurate to a fraction of eps) and publishes it from its last
+// workgroup: each workgroup ORs its verdict into sync[0] and takes a ticket from sync[1]; the
+// one with the last ticket moves sync[0] into the flag and sets both words back to zero, so
+// every launch leaves them as it found them (zero) and no memset is needed between calls
+// (graph capture records none).
 template <typename T>
-__global__ void __launch_bounds__(256) k_check_norm(const T* __restrict__ cs, int64_t ld,
-                                                    int n, int k, int* __restrict__ flag)
+__device__ __forceinline__ bool off_band(T c, T s)
 {
-    const int64_t total = (int64_t)(n - 1) * k;
     constexpr T kBand = T(1) / T(64);
-    int bad = 0;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
-         q += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = q % (n - 1), L = q / (n - 1);
-        const T c = cs[2 * (i + ld * L)], s = cs[2 * (i + ld * L) + 1];
-        bad |= !(fabs(fma(c, c, fma(s, s, T(-1)))) <= kBand);
-    }
-    if (__any(bad) && (threadIdx.x % kLanes) == 0) atomicOr(flag, 1);
+    return !(fabs(fma(c, c, fma(s, s, T(-1)))) <= kBand);
 }
 
-// Clears the normalisation flag.  A kernel rather than hipMemsetAsync, so that the launch
-// sequence holds no memset node when a caller captures it into a graph.
-__global__ void __launch_bounds__(64) k_zero_flag(int* __restrict__ flag)
+// Zeroes the two sync words of a freshly allocated workspace (a kernel rather than
+// hipMemsetAsync, so that a captured launch sequence holds no memset node).
+__global__ void __launch_bounds__(64) k_zero_sync(unsigned* __restrict__ sync)
 {
-    if (threadIdx.x == 0) flag[0] = 0;
+    if (threadIdx.x < 2) sync[threadIdx.x] = 0u;
 }
 
 // Layers of one pass of every geometry (the scale table holds this many per column).
@@ -102,7 +98,8 @@ struct ApplyMem {
     T* table;
     T* escale;  // [n][kMaxPassLayers]: column scale before each layer of the pass
     T* sigm1;   // [n]: the column's scale factor at the pass boundary, minus one
-    int* flag;
+    int* flag;  // the pass's normalisation flag
+    unsigned* sync;  // k_colscale's two completion words (zero between launches)
 };
 
 // The auxiliary words of an n-column apply, laid out from `aux` (256-byte aligned).
@@ -111,11 +108,12 @@ inline size_t aux_bytes(int64_t n, size_t tsize)
     return tsize * (size_t)n * (kMaxPassLayers + 1) + 256;
 }
 template <typename T>
-ApplyMem<T> apply_mem(void* table, void* aux, int64_t n)
+ApplyMem<T> apply_mem(void* table, void* aux, int64_t n, void* sync)
 {
     T* e = static_cast<T*>(aux);
     T* sigm1 = e + (size_t)n * kMaxPassLayers;
-    return ApplyMem<T>{static_cast<T*>(table), e, sigm1, reinterpret_cast<int*>(sigm1 + n)};
+    return ApplyMem<T>{static_cast<T*>(table), e, sigm1, reinterpret_cast<int*>(sigm1 + n),
+                       static_cast<unsigned*>(sync)};
 }
 
 // The reference's identity skip: bulk_wave_order_2x2_rmul! skips a fused block of four
@@ -148,7 +146,8 @@ __device__ __forceinline__ bool rot_skipped(const T* __restrict__ cs, int64_t ld
 // stored values it is  a^' = C a^ + S r b^,  b^' = -(S / r) a^ + C b^  with r = e_a / e_b,
 // a matrix of determinant C^2 + S^2 = 1, which three shears realise exactly:
 //   a^ += p b^;  b^ += q a^;  a^ += p b^   with  p = S r / (1 + C),  q = -S / r.
-// |p| <= |r| and |q| <= 1/|r|, and r stays near 1 inside the band k_check_norm admits.
+// |p| <= |r| and |q| <= 1/|r|, and r stays near 1 inside the band the normalisation check
+// admits.
 // A plain rotation has rho = 1 and e = +-1: that is sign tracking.  Grids whose pairs are
 // only near normalised thus cost no accuracy: the determinant 1 + delta of each rotation
 // goes into the scales instead of being dropped.  Skipped and out-of-grid slots have
@@ -159,12 +158,17 @@ __device__ __forceinline__ T rho_signed(T c, T s)
     const T rho = sqrt(fma(c, c, s * s));
     return c < T(0) ? -rho : rho;
 }
+// The rho_s applied by slot (i, L), from its pair (c, s) already loaded (in: the slot lies in
+// the grid): 1 outside the grid and for a skipped rotation.  Only a pair with s = 0 and
+// c != 1 (whose rho_s is c, not 1, unless its fused block is skipped) reads the skip rule's
+// other entries.
 template <typename T>
-__device__ __forceinline__ T applied_rho(const T* __restrict__ cs, int64_t ld, int n, int64_t i,
-                                         int L, int k)
+__device__ __forceinline__ T applied_rho_of(const T* __restrict__ cs, int64_t ld, int n,
+                                            int64_t i, int L, int k, bool in, T c, T s)
 {
-    if (i < 0 || i > n - 2 || L >= k || rot_skipped<T>(cs, ld, n, i, L, k)) return T(1);
-    return rho_signed<T>(cs[2 * (i + ld * L)], cs[2 * (i + ld * L) + 1]);
+    if (!in) return T(1);
+    if (s == T(0) && c != T(1) && rot_skipped<T>(cs, ld, n, i, L, k)) return T(1);
+    return rho_signed<T>(c, s);
 }
 
 // Column scales of one pass (layers L0 .. L0 + Kp - 1).  d_j(lr) is the product of the
@@ -172,59 +176,118 @@ __device__ __forceinline__ T applied_rho(const T* __restrict__ cs, int64_t ld, i
 // in each layer), D_j = d_j(Kp).  The stored column is x^ = e x with e = D_j / d_j when
 // in_fix (the kernel multiplies the column by D_j as it enters the pass, and it leaves
 // with e = 1), and e = 1 / d_j otherwise (it enters unscaled, and the kernel multiplies it
-// by D_j as it leaves).  escale[j][lr] = e before layer L0 + lr, sigm1[j] = D_j - 1 (zero
-// on the five-instruction path, which tracks nothing).
+// by D_j as it leaves).  escale[j][lr] = e before layer L0 + lr, sigm1[j] = D_j - 1 (not
+// read on the five-instruction path, which tracks nothing).
 template <typename T>
-__global__ void __launch_bounds__(256) k_colscale(const T* __restrict__ cs, int64_t ld, int n,
-                                                  int k, int L0, int Kp, int in_fix,
-                                                  const int* __restrict__ flag,
-                                                  T* __restrict__ escale, T* __restrict__ sigm1)
+__global__ void __launch_bounds__(1024) k_colscale(const T* __restrict__ cs, int64_t ld, int n,
+                                                   int k, int L0, int Kp, int in_fix, int check,
+                                                   unsigned* __restrict__ sync,
+                                                   int* __restrict__ flag,
+                                                   T* __restrict__ escale, T* __restrict__ sigm1)
 {
-    // 64 columns per block, four threads per column, each over 16 of the (at most 64)
-    // layers: the running product within its chunk, then the chunks' products in order
-    __shared__ T cp[4][64];
+    // 64 columns per block, sixteen threads per column, each over 4 of the (at most 64)
+    // layers: the running product within its chunk, then the chunks' products in order.
+    // The scales go through an LDS tile so that the block writes its 64 x 64 region of
+    // escale with consecutive lanes on consecutive layers.
+    constexpr int kChunks = 16, kPer = kMaxPassLayers / kChunks;
+    __shared__ T cp[kChunks][64];
+    __shared__ T tile[64][kMaxPassLayers + 1];
     const int c = threadIdx.x % 64, q = threadIdx.x / 64;
-    const int j = blockIdx.x * 64 + c;
-    const bool general = *flag != 0;
-    T part[16];  // product of the chunk's layers before layer q 16 + u
+    const int j0 = blockIdx.x * 64, j = j0 + c;
+    // every pair of the chunk is loaded first (clamped in-grid addresses), so that all the
+    // loads are in flight together.  The scales are computed whatever the flag says (the
+    // five-instruction path reads none of them).
+    T ca[kPer], sa[kPer], cb[kPer], sb[kPer];
+    bool ia[kPer], ib[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int lr = q * kPer + u, L = L0 + lr;
+        const bool use = j < n && lr < Kp && L < k;
+        ia[u] = use && j >= 1;
+        ib[u] = use && j <= n - 2;
+        const int64_t a = ia[u] ? 2 * ((j - 1) + ld * L) : 0;
+        const int64_t b = ib[u] ? 2 * (j + ld * L) : 0;
+        ca[u] = cs[a];
+        sa[u] = cs[a + 1];
+        cb[u] = cs[b];
+        sb[u] = cs[b + 1];
+    }
+    T part[kPer];  // product of the chunk's layers before layer q kPer + u
     T run = T(1);
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-        const int lr = q * 16 + u, L = L0 + lr;
+    for (int u = 0; u < kPer; ++u) {
+        const int L = L0 + q * kPer + u;
         part[u] = run;
-        if (j < n && !general && lr < Kp && L < k)
-            run *= applied_rho<T>(cs, ld, n, j - 1, L, k) * applied_rho<T>(cs, ld, n, j, L, k);
+        run *= applied_rho_of<T>(cs, ld, n, j - 1, L, k, ia[u], ca[u], sa[u]) *
+               applied_rho_of<T>(cs, ld, n, j, L, k, ib[u], cb[u], sb[u]);
+    }
+    // the normalisation check, on the pairs (j, L) of columns j = 0 .. n-2: every pair once
+    bool bad = false;
+    if (check) {
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) bad |= ib[u] && off_band<T>(cb[u], sb[u]);
+    }
+    // The words live in L2 (device-coherent atomics).  A wave waits for its OR to return
+    // before the barrier, so every verdict of the workgroup is in sync[0] before thread 0
+    // takes the ticket, and the last ticket sees all of them.
+    if (__any(bad) && (threadIdx.x % 64) == 0) {
+        const unsigned old = atomicOr(&sync[0], 1u);
+        asm volatile("" ::"v"(old));
     }
     cp[q][c] = run;
     __syncthreads();
-    if (j >= n) return;
+    if (threadIdx.x == 0) {
+        if (atomicAdd(&sync[1], 1u) == gridDim.x - 1) {
+            *flag = (int)atomicExch(&sync[0], 0u);
+            atomicExch(&sync[1], 0u);
+        }
+    }
     T pre = T(1), D = T(1);
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < kChunks; ++t) {
         if (t < q) pre *= cp[t][c];
         D *= cp[t][c];
     }
-    if (general) D = T(1);
     const T num = in_fix ? D : T(1);
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-        const int lr = q * 16 + u;
-        if (lr < kMaxPassLayers) escale[(int64_t)j * kMaxPassLayers + lr] = num / (pre * part[u]);
+    for (int u = 0; u < kPer; ++u) tile[c][q * kPer + u] = num / (pre * part[u]);
+    if (q == 0 && j < n) sigm1[j] = D - T(1);
+    __syncthreads();
+    T* __restrict__ out = escale + (int64_t)j0 * kMaxPassLayers;
+    for (int e = threadIdx.x; e < 64 * kMaxPassLayers; e += blockDim.x) {
+        const int col = e / kMaxPassLayers;
+        if (j0 + col < n) out[e] = tile[col][e % kMaxPassLayers];
     }
-    if (q == 0) sigm1[j] = D - T(1);
 }
 
-// Shear coefficients (p, q) of rotation (i, L) = layer L0 + lr, applied, from the scales.
+// Table entries of slot (i, L) = layer L0 + lr: (0, 0) outside the grid and for a skipped
+// rotation (rot_skipped), (c - 1, s) on the five-instruction path, else the shear
+// coefficients (p, q) of the scaled form from the column scales.  Every load is issued
+// first, from clamped in-grid addresses, so that they are all in flight together; only a
+// pair with s = 0 reads the skip rule's other entries.
 template <typename T>
-__device__ __forceinline__ void shear_coefs(const T* __restrict__ cs, int64_t ld, int n, int64_t i,
-                                            int L, int k, int lr, const T* __restrict__ escale,
-                                            T& p, T& qq)
+__device__ __forceinline__ void slot_coefs(const T* __restrict__ cs, int64_t ld, int n,
+                                           int64_t i, int L, int k, int lr, int general,
+                                           const T* __restrict__ escale, T& p, T& qq)
 {
-    const T c = cs[2 * (i + ld * L)], s = cs[2 * (i + ld * L) + 1];
+    const bool in = L < k && i >= 0 && i <= n - 2;
+    const bool inm = in && i >= 1;
+    const int64_t at = in ? 2 * (i + ld * L) : 0, atm = inm ? 2 * (i - 1 + ld * L) : 0;
+    const int64_t ie = in ? i : 0;
+    const T c = cs[at], s = cs[at + 1], cm = cs[atm], sm = cs[atm + 1];
+    // scale of column i before layer L, of column i + 1 before layer L
+    const T e0 = escale[ie * kMaxPassLayers + lr], eb = escale[(ie + 1) * kMaxPassLayers + lr];
+    p = T(0);
+    qq = T(0);
+    if (!in || (s == T(0) && rot_skipped<T>(cs, ld, n, i, L, k))) return;
+    if (general) {
+        p = c - T(1);
+        qq = s;
+        return;
+    }
     const T rs = rho_signed<T>(c, s);
     const T C = fabs(c) / fabs(rs), S = s / rs;
-    // scale of column i after rotation (i-1, L) touched it, of column i+1 before layer L
-    const T ea = escale[i * kMaxPassLayers + lr] / applied_rho<T>(cs, ld, n, i - 1, L, k);
-    const T eb = escale[(i + 1) * kMaxPassLayers + lr];
+    // scale of column i after rotation (i-1, L) touched it
+    const T ea = e0 / applied_rho_of<T>(cs, ld, n, i - 1, L, k, inm, cm, sm);
     const T r = ea / eb;
     p = S * r / (T(1) + C);
     qq = -S / r;
@@ -268,16 +331,8 @@ __global__ void __launch_bounds__(256) k_coef_table(T* __restrict__ tab,
         const int lr = g * KG + l;
         const int L = L0 + lr;
         const int64_t i = (int64_t)(u - S - PADU) - 2 * (int64_t)lr;
-        T p = 0.0, qq = 0.0;
-        if (L < k && i >= 0 && i <= n - 2 && !rot_skipped<T>(cs, ld, n, i, L, k)) {
-            const T c = cs[2 * (i + ld * L)], s = cs[2 * (i + ld * L) + 1];
-            if (general) {
-                p = c - 1.0;
-                qq = s;
-            } else {
-                shear_coefs<T>(cs, ld, n, i, L, k, lr, escale, p, qq);
-            }
-        }
+        T p, qq;
+        slot_coefs<T>(cs, ld, n, i, L, k, lr, general, escale, p, qq);
         tab[2 * q] = p;
         tab[2 * q + 1] = qq;
     }
@@ -717,20 +772,12 @@ int run_dpp(sep_handle* th, hipStream_t st, const ApplyMem<T>& mem, int64_t m, i
     T* sigm1 = mem.sigm1;
     int* flag = mem.flag;
     const int tp0 = timing_mark(h, st);
-    hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(64), 0, st, flag);
-    SEP_CHECK_LAUNCH();
-    if (!rotations) {
-        const int64_t tot = (n - 1) * k;
-        const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 2048);
-        hipLaunchKernelGGL(k_check_norm<T>, dim3(blocks), dim3(256), 0, st, cs, ld_cs, (int)n,
-                           (int)k, flag);
-        SEP_CHECK_LAUNCH();
-    }
     const int64_t rb = (m + kLanes - 1) / kLanes;
     int tp_prev = tp0;
     for (int64_t L0 = 0; L0 < k; L0 += Kp) {
-        hipLaunchKernelGGL(k_colscale<T>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st, cs,
-                           ld_cs, (int)n, (int)k, (int)L0, Kp, 1, flag, escale, sigm1);
+        hipLaunchKernelGGL(k_colscale<T>, dim3((unsigned)((n + 63) / 64)), dim3(1024), 0, st, cs,
+                           ld_cs, (int)n, (int)k, (int)L0, Kp, 1, rotations ? 0 : 1, mem.sync,
+                           flag, escale, sigm1);
         SEP_CHECK_LAUNCH();
         const int64_t total = (int64_t)G * Up * KG + (int64_t)NI * 16;
         const int tblocks = (int)std::min<int64_t>((total + 255) / 256, 8192);
@@ -831,18 +878,9 @@ __global__ void __launch_bounds__(256) k_coef_r16(T* __restrict__ tab,
         const int lr = q * KG + (j % NC) / 2;
         const int L = L0 + lr;
         const int64_t i = u - (int64_t)q * (S - 1) - 2 * lr;
-        T v = 0.0;
-        if (L < k && i >= 0 && i <= n - 2 && !rot_skipped<T>(cs, ld, n, i, L, k)) {
-            const T c = cs[2 * (i + ld * L)], s = cs[2 * (i + ld * L) + 1];
-            if (general) {
-                v = (j & 1) ? s : c - 1.0;
-            } else {
-                T p, qq;
-                shear_coefs<T>(cs, ld, n, i, L, k, lr, escale, p, qq);
-                v = (j & 1) ? qq : p;
-            }
-        }
-        tab[e] = v;
+        T p, qq;
+        slot_coefs<T>(cs, ld, n, i, L, k, lr, general, escale, p, qq);
+        tab[e] = (j & 1) ? qq : p;
     }
 }
 
@@ -873,7 +911,8 @@ __device__ __forceinline__ void r16_interval(const R16Args<T>& a, T (*buf)[2][Gm
         for (int k4 = 0; k4 < NQ; ++k4) {
             ob[k4] = buf[NW][1 - P][GPW * k4 + r][OL + j];
             const int oc = oc0 + GPW * k4 + r;
-            sg[k4] = (oful || (oc >= 0 && oc < a.n)) ? a.sigm1[oc] : 0.0;
+            // the five-instruction path tracks no scales (its sigm1 is not computed for it)
+            sg[k4] = (!GENERAL && (oful || (oc >= 0 && oc < a.n))) ? a.sigm1[oc] : 0.0;
         }
     }
     Steps<S, 0>::run([&](auto sc) {
@@ -990,7 +1029,7 @@ __device__ __forceinline__ void r16_body(T* __restrict__ A, int64_t lda, int m, 
             const int oc = (NB - 1) * S + GPW * k4 + r - Gm::DL;
             if (oc >= 0 && oc < n) {
                 const T x = buf[NW][1][GPW * k4 + r][OL + j];
-                a.A0[(int64_t)(oc - r) * lda + a.voff] = fma(sigm1[oc], x, x);
+                a.A0[(int64_t)(oc - r) * lda + a.voff] = GENERAL ? x : fma(sigm1[oc], x, x);
             }
         }
     }
@@ -1036,20 +1075,12 @@ int run_r16(sep_handle* th, hipStream_t st, const ApplyMem<T>& mem, int64_t m, i
     T* sigm1 = mem.sigm1;
     int* flag = mem.flag;
     const int tp0 = timing_mark(h, st);
-    hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(64), 0, st, flag);
-    SEP_CHECK_LAUNCH();
-    if (!rotations) {
-        const int64_t tot = (n - 1) * k;
-        const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 2048);
-        hipLaunchKernelGGL(k_check_norm<T>, dim3(blocks), dim3(256), 0, st, cs, ld_cs, (int)n,
-                           (int)k, flag);
-        SEP_CHECK_LAUNCH();
-    }
     const int64_t wgs = (m + Gm::RPG - 1) / Gm::RPG;
     int tp_prev = tp0;
     for (int64_t L0 = 0; L0 < k; L0 += Kp) {
-        hipLaunchKernelGGL(k_colscale<T>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st, cs,
-                           ld_cs, (int)n, (int)k, (int)L0, Kp, 0, flag, escale, sigm1);
+        hipLaunchKernelGGL(k_colscale<T>, dim3((unsigned)((n + 63) / 64)), dim3(1024), 0, st, cs,
+                           ld_cs, (int)n, (int)k, (int)L0, Kp, 0, rotations ? 0 : 1, mem.sync,
+                           flag, escale, sigm1);
         SEP_CHECK_LAUNCH();
         const int tblocks = (int)std::min<int64_t>(((int64_t)tab_elems + 255) / 256, 8192);
         hipLaunchKernelGGL((k_coef_r16<T, Gm>), dim3(tblocks), dim3(256), 0, st, mem.table, cs,
@@ -1137,6 +1168,7 @@ int check_apply_range(int64_t m, int64_t n)
 }
 
 constexpr size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kSyncBytes = 256;  // k_colscale's sync words at the head of the scratch
 
 }  // namespace
 
@@ -1149,8 +1181,13 @@ int bulk_apply_dev(sep_handle* h, hipStream_t st, ApplyWork& w, int64_t m, int64
     const int geom = choose_geom<T>(h, m, k);
     if (int rc = w.table.reserve(table_bytes<T>(geom, n))) return rc;
     if (int rc = w.aux.reserve(aux_bytes(n, sizeof(T)))) return rc;
-    return launch_geom<T>(geom, h, st, apply_mem<T>(w.table.p, w.aux.p, n), m, n, A, lda, cs,
-                          ld_cs, k, rotations);
+    if (!w.sync.p) {
+        if (int rc = w.sync.reserve(256)) return rc;
+        hipLaunchKernelGGL(k_zero_sync, dim3(1), dim3(64), 0, st, w.sync.as<unsigned>());
+        SEP_CHECK_LAUNCH();
+    }
+    return launch_geom<T>(geom, h, st, apply_mem<T>(w.table.p, w.aux.p, n, w.sync.p), m, n, A,
+                          lda, cs, ld_cs, k, rotations);
 }
 
 template int bulk_apply_dev<double>(sep_handle*, hipStream_t, ApplyWork&, int64_t, int64_t,
@@ -1205,18 +1242,20 @@ int bulk_apply_noalloc_f64(sep_handle* h, int64_t m, int64_t n, double* A, int64
         return SEP_ERR_HIP;
     }
     const int geom = choose_geom<double>(h, m, k);
+    // layout: k_colscale's sync words (zero since the array was loaded, and left zero by
+    // every launch), the coefficient table, the auxiliary words
     const size_t tb = round256(table_bytes<double>(geom, n));
-    const size_t need = tb + aux_bytes(n, sizeof(double));
+    const size_t need = kSyncBytes + tb + aux_bytes(n, sizeof(double));
     if (need > kNoAllocScratchBytes) {
         set_error("sep_bulk_rmul_f64_dev_noalloc: n=%lld needs %zu bytes of scratch, above the "
                   "fixed %zu (use sep_bulk_rmul_f64_dev)",
                   (long long)n, need, (size_t)kNoAllocScratchBytes);
         return SEP_ERR_ARG;
     }
-    unsigned char* b = static_cast<unsigned char*>(base);
+    unsigned char* b = static_cast<unsigned char*>(base) + kSyncBytes;
     // no timing events: the instrumentation may create them
-    return launch_geom<double>(geom, nullptr, h->stream, apply_mem<double>(b, b + tb, n), m, n,
-                               A, lda, cs, n - 1, k, false);
+    return launch_geom<double>(geom, nullptr, h->stream, apply_mem<double>(b, b + tb, n, base), m,
+                               n, A, lda, cs, n - 1, k, false);
 }
 
 

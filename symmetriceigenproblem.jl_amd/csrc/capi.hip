@@ -283,6 +283,7 @@ int sep_destroy(sep_handle* h)
     for (auto& w : h->apply_work) {
         w.table.release();
         w.aux.release();
+        w.sync.release();
     }
     for (auto& g : h->grid) g.release();
     h->state.release();

@@ -69,7 +69,8 @@ struct ApplyGeom {
 // Per-launch workspace of the systolic apply: the relaid (c,s) sweep table.
 struct ApplyWork {
     DevBuf table;
-    DevBuf aux;  // column sign masks, output signs, normalisation flag
+    DevBuf aux;   // column scales, pass scale factors, normalisation flag
+    DevBuf sync;  // the column-scale kernel's completion words (zeroed once, left zero)
 };
 
 }  // namespace sep
