@@ -184,6 +184,44 @@ __device__ void block_max_sum(double& mx, double& sq, double* red, double* red2)
     }
 }
 
+// The reflector H = I - tau v v^T with H (alpha, x) = (beta, 0), v = (1, x / (alpha - beta)),
+// as LAPACK's dlarfg: beta = -sign(alpha) hypot(alpha, |x|), tau = (beta - alpha) / beta.
+// Like dlarfg, a |beta| below safmin = 2^-969 (x and alpha near the underflow threshold,
+// where 1 / (alpha - beta) would lose precision or overflow) is recomputed from alpha and x
+// scaled by 2^969, up to 20 times; v is formed from the scaled x and beta scaled back.
+// |x| = unit sqrt(ssn) (unit a power of two, ssn = sum (x_j / unit)^2) scales exactly with x.
+struct Refl {
+    double beta, tau, sc;
+    int knt;
+    __device__ double scaled(double x) const
+    {
+        for (int t = 0; t < knt; ++t) x *= 0x1p969;
+        return x * sc;
+    }
+};
+__device__ __forceinline__ Refl reflector(double alpha, double xnorm, double unit, double ssn)
+{
+    Refl o;
+    double beta = -copysign(hypot(alpha, xnorm), alpha);
+    o.knt = 0;
+    if (fabs(beta) < 0x1p-969) {
+        double a = alpha, m = unit;
+        do {
+            ++o.knt;
+            beta *= 0x1p969;
+            a *= 0x1p969;
+            m *= 0x1p969;
+        } while (fabs(beta) < 0x1p-969 && o.knt < 20);
+        beta = -copysign(hypot(a, m * sqrt(ssn)), a);
+        alpha = a;
+    }
+    o.tau = (beta - alpha) / beta;
+    o.sc = 1.0 / (alpha - beta);
+    for (int t = 0; t < o.knt; ++t) beta *= 0x1p-969;
+    o.beta = beta;
+    return o;
+}
+
 // Column i: finish the previous column's w (from y = A22 v of column i - 1), apply its
 // update to column i, write d[i] and generate column i's reflector into A, vn, e, tau.
 // Rows [i, n) are held in registers (kPer per thread) when they fit: one round of loads,
@@ -252,20 +290,24 @@ __global__ void __launch_bounds__(kVecThreads)
             }
         }
         block_max_sum(mx, sq, red, red2);
-        double xnorm;
+        // scaled branch: |x| = 2^ex sqrt(ssn), ssn the sum of squares of x / 2^ex with
+        // 2^ex <= mx < 2^(ex+1) (exact power-of-two scaling; 1 / mx would overflow for a
+        // subnormal mx)
+        double xnorm, ssn = 1.0, unit = 1.0;
         if (mx == 0.0 || (mx > 0x1p-400 && mx < 0x1p400)) {
             xnorm = sqrt(sq);
         } else {
-            const double inv = 1.0 / mx;
+            const int ex = ilogb(mx);
             double ss = 0.0;
 #pragma unroll
             for (int k = 0; k < kPer; ++k) {
                 const int r = i + threadIdx.x + k * kVecThreads;
-                const double t = (r >= i + 2 && r < n) ? cv[k] * inv : 0.0;
+                const double t = (r >= i + 2 && r < n) ? ldexp(cv[k], -ex) : 0.0;
                 ss += t * t;
             }
-            ss = block_sum(ss, red);
-            xnorm = mx * sqrt(ss);
+            ssn = block_sum(ss, red);
+            unit = ldexp(1.0, ex);
+            xnorm = ldexp(sqrt(ssn), ex);
         }
         if (xnorm == 0.0) {
 #pragma unroll
@@ -282,22 +324,22 @@ __global__ void __launch_bounds__(kVecThreads)
             }
             return;
         }
-        const double beta = -copysign(hypot(alpha, xnorm), alpha);
-        const double sc = 1.0 / (alpha - beta);
+        // (a beta below 2^-969 comes only from the scaled branch: mx <= 2^-400 there)
+        const Refl rf = reflector(alpha, xnorm, unit, ssn);
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int r = i + threadIdx.x + k * kVecThreads;
             if (r >= i + 2 && r < n) {
-                const double v = cv[k] * sc;
+                const double v = rf.scaled(cv[k]);
                 col[r] = v;
                 vn[r] = v;
             }
         }
         if (threadIdx.x == 0) {
             vn[i + 1] = 1.0;
-            col[i + 1] = beta;
-            e[i] = beta;
-            tau[i] = (beta - alpha) / beta;
+            col[i + 1] = rf.beta;
+            e[i] = rf.beta;
+            tau[i] = rf.tau;
         }
         return;
     }
@@ -323,16 +365,17 @@ __global__ void __launch_bounds__(kVecThreads)
     double mx = 0.0;
     for (int r = i + 2 + threadIdx.x; r < n; r += blockDim.x) mx = fmax(mx, fabs(col[r]));
     mx = block_max(mx, red);
+    // |x| = 2^ex sqrt(ss), ss the sum of squares of x / 2^ex, 2^ex <= mx < 2^(ex+1)
     double ss = 0.0;
+    const int ex = mx > 0.0 ? ilogb(mx) : 0;
     if (mx > 0.0) {
-        const double inv = 1.0 / mx;
         for (int r = i + 2 + threadIdx.x; r < n; r += blockDim.x) {
-            const double t = col[r] * inv;
+            const double t = ldexp(col[r], -ex);
             ss += t * t;
         }
     }
     ss = block_sum(ss, red);
-    const double xnorm = mx * sqrt(ss);
+    const double xnorm = mx > 0.0 ? ldexp(sqrt(ss), ex) : 0.0;
     if (xnorm == 0.0) {
         for (int r = i + 1 + threadIdx.x; r < n; r += blockDim.x) vn[r] = r == i + 1 ? 1.0 : 0.0;
         if (threadIdx.x == 0) {
@@ -341,19 +384,18 @@ __global__ void __launch_bounds__(kVecThreads)
         }
         return;
     }
-    const double beta = -copysign(hypot(alpha, xnorm), alpha);
-    const double sc = 1.0 / (alpha - beta);
+    const Refl rf = reflector(alpha, xnorm, ldexp(1.0, ex), ss);
     __syncthreads();  // every thread has read col[i + 1] and x before they change
     for (int r = i + 2 + threadIdx.x; r < n; r += blockDim.x) {
-        const double v = col[r] * sc;
+        const double v = rf.scaled(col[r]);
         col[r] = v;
         vn[r] = v;
     }
     if (threadIdx.x == 0) {
         vn[i + 1] = 1.0;
-        col[i + 1] = beta;
-        e[i] = beta;
-        tau[i] = (beta - alpha) / beta;
+        col[i + 1] = rf.beta;
+        e[i] = rf.beta;
+        tau[i] = rf.tau;
     }
 }
 

@@ -154,3 +154,29 @@ def test_dense_long_columns_on_device(sep, handle):
     assert torch.linalg.norm(Q.T @ Q - I).item() <= 4e-14 * n
     T = torch.diag(d) + torch.diag(e, 1) + torch.diag(e, -1)
     assert torch.linalg.norm(Q.T @ (A @ Q) - T).item() <= 4e-14 * n * nA
+
+
+@pytest.mark.parametrize("scale_exp", [-1000, -1040])
+def test_sytrd_near_underflow(sep, handle, scale_exp):
+    """A scaled by 2^-1000 (normal entries whose reflector beta lies below dlarfg's safmin,
+    2^-969) and 2^-1040 (subnormal entries, where 1 / (alpha - beta) overflows unless x and
+    alpha are rescaled first): like LAPACK's dlarfg, the reflector is recomputed from x and
+    alpha scaled by 2^969.  Finite factors, an orthogonal Q, and d, e, tau against LAPACK
+    dsytrd of the same matrix: 1e-12 relative at 2^-1000; at 2^-1040 the subnormal inputs
+    carry ~34 bits, so both reductions agree to 1e-5 (LAPACK itself moves by 2.4e-7
+    against the unscaled reduction there)."""
+    n = 80
+    A0 = sym_matrix(n, seed=33)
+    A = np.ldexp(A0, scale_exp)
+    _, d_ref, e_ref, tau_ref = lapack_sytrd(A)
+    G = A.copy(order="F")
+    d, e, tau = sep.sytrd_(G, handle=handle)
+    assert np.all(np.isfinite(d)) and np.all(np.isfinite(e)) and np.all(np.isfinite(tau))
+    assert np.all(np.isfinite(G))
+    tol = 1e-12 if scale_exp >= -1000 else 1e-5
+    nA = np.ldexp(np.linalg.norm(A0), scale_exp)
+    assert np.abs(d - d_ref).max() <= tol * nA
+    assert np.abs(e - e_ref).max() <= tol * nA
+    assert np.abs(tau - tau_ref).max() <= tol * 10
+    Q = sep.orgtr(G, tau, handle=handle)
+    assert np.linalg.norm(Q.T @ Q - np.eye(n)) <= 1e-14 * n * 4
