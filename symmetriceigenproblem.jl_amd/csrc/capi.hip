@@ -37,6 +37,16 @@ int DevBuf::reserve(size_t want)
     return SEP_OK;
 }
 
+// Make the handle's device current.  hipSetDevice is skipped when it already is: a caller
+// capturing a stream into a graph (global or thread-local capture mode) may not call it.
+static int use_device(const sep_handle* h)
+{
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur == h->device) return SEP_OK;
+    SEP_HIP(hipSetDevice(h->device));
+    return SEP_OK;
+}
+
 int timing_mark(sep_handle* h, hipStream_t st)
 {
     if (!h || !h->timing) return -1;
@@ -81,7 +91,7 @@ int bulk_rmul_dev_checked(sep_handle* h, int64_t m, int64_t n, T* A, int64_t lda
         set_error("sep_bulk_rmul: null pointer");
         return SEP_ERR_ARG;
     }
-    SEP_HIP(hipSetDevice(h->device));
+    if (int rc = use_device(h)) return rc;
     return bulk_apply_dev<T>(h, h->stream, h->apply_work[0], m, n, A, lda, cs, n - 1, k);
 }
 
@@ -101,7 +111,7 @@ int bulk_rmul_host(sep_handle* h, int64_t m, int64_t n, T* A, int64_t lda, const
         set_error("sep_bulk_rmul: null pointer");
         return SEP_ERR_ARG;
     }
-    SEP_HIP(hipSetDevice(h->device));
+    if (int rc = use_device(h)) return rc;
     const size_t abytes = sizeof(T) * (size_t)m * (size_t)n;
     const size_t gbytes = sizeof(T) * 2 * (size_t)(n - 1) * (size_t)k;
     if (int rc = h->host_io_A.reserve(abytes)) return rc;
@@ -145,7 +155,7 @@ int qr_checked(sep_handle* h, int64_t n, T* dv, T* ev, T* Q, int64_t ldq, int64_
         set_error("sep_qr_algorithm: ldq=%lld < n=%lld", (long long)ldq, (long long)n);
         return SEP_ERR_ARG;
     }
-    SEP_HIP(hipSetDevice(h->device));
+    if (int rc = use_device(h)) return rc;
     return qr_algorithm_dev<T>(h, n, dv, ev, Q, ldq, max_layers, tol, maxiter, converged, iter,
                                n_rot);
 }
@@ -178,7 +188,7 @@ int qr_host(sep_handle* h, int64_t n, T* dv, T* ev, T* Q, int64_t ldq, int64_t m
         if (n_rot) *n_rot = 0;
         return SEP_OK;
     }
-    SEP_HIP(hipSetDevice(h->device));
+    if (int rc = use_device(h)) return rc;
     if (int rc = h->host_io_d.reserve(sizeof(T) * (size_t)n)) return rc;
     if (int rc = h->host_io_e.reserve(sizeof(T) * (size_t)(n > 1 ? n - 1 : 1))) return rc;
     T* dd = h->host_io_d.as<T>();
@@ -326,7 +336,7 @@ int sep_set_timing(sep_handle* h, int enable)
         set_error("sep_set_timing: null handle");
         return SEP_ERR_ARG;
     }
-    SEP_HIP(hipSetDevice(h->device));
+    if (int rc = use_device(h)) return rc;
     SEP_HIP(hipStreamSynchronize(h->stream));
     h->timing = enable != 0;
     h->tapply.clear();
@@ -341,7 +351,7 @@ int sep_get_timing(sep_handle* h, double* apply_ms, double* prep_ms, int64_t* la
         set_error("sep_get_timing: null handle");
         return SEP_ERR_ARG;
     }
-    SEP_HIP(hipSetDevice(h->device));
+    if (int rc = use_device(h)) return rc;
     SEP_HIP(hipStreamSynchronize(h->stream));
     double a = 0, p = 0;
     for (auto& pr : h->tapply) {
@@ -366,7 +376,7 @@ int sep_set_stream(sep_handle* h, void* stream)
         set_error("sep_set_stream: null handle");
         return SEP_ERR_ARG;
     }
-    SEP_HIP(hipSetDevice(h->device));
+    if (int rc = use_device(h)) return rc;
     if (h->own_stream && h->stream) SEP_HIP(hipStreamSynchronize(h->stream));
     if (stream) {
         if (h->own_stream && h->stream) SEP_HIP(hipStreamDestroy(h->stream));
@@ -385,7 +395,7 @@ int sep_synchronize(sep_handle* h)
         set_error("sep_synchronize: null handle");
         return SEP_ERR_ARG;
     }
-    SEP_HIP(hipSetDevice(h->device));
+    if (int rc = use_device(h)) return rc;
     SEP_HIP(hipStreamSynchronize(h->stream));
     SEP_HIP(hipStreamSynchronize(h->chase_stream));
     return SEP_OK;
@@ -423,7 +433,7 @@ int sep_bulk_rmul_f64_dev_noalloc(sep_handle* h, int64_t m, int64_t n, double* A
         set_error("sep_bulk_rmul_f64_dev_noalloc: null pointer");
         return SEP_ERR_ARG;
     }
-    SEP_HIP(hipSetDevice(h->device));
+    if (int rc = use_device(h)) return rc;
     return bulk_apply_noalloc_f64(h, m, n, A, lda, cs, k);
 }
 int sep_bulk_rmul_f32_dev(sep_handle* h, int64_t m, int64_t n, float* A, int64_t lda,
@@ -469,7 +479,7 @@ int sep_qr_batched_f64_dev(sep_handle* h, int64_t B, const int64_t* n, const int
         set_error("sep_qr_batched: null handle");
         return SEP_ERR_ARG;
     }
-    SEP_HIP(hipSetDevice(h->device));
+    if (int rc = use_device(h)) return rc;
     return qr_batched_f64(h, B, n, off_d, off_q, dv, ev, Q, max_layers, converged, iters);
 }
 
@@ -481,7 +491,7 @@ int sep_tune_batched_f64(sep_handle* h, int64_t B, const int64_t* n, const int64
         set_error("sep_tune_batched: null handle");
         return SEP_ERR_ARG;
     }
-    SEP_HIP(hipSetDevice(h->device));
+    if (int rc = use_device(h)) return rc;
     return tune_batched_f64(h, B, n, off_d, off_q, dv, ev, choice);
 }
 
@@ -527,7 +537,7 @@ int sep_eig_backward_f64_dev(sep_handle* h, int64_t n, const double* lam, const 
         set_error("sep_eig_backward: null handle");
         return SEP_ERR_ARG;
     }
-    SEP_HIP(hipSetDevice(h->device));
+    if (int rc = use_device(h)) return rc;
     return eig_backward_f64(h, n, lam, Q, ldq, lam_bar, Q_bar, d_bar, e_bar);
 }
 
@@ -540,7 +550,7 @@ int sep_eig_backward_batched_f64_dev(sep_handle* h, int64_t B, const int64_t* n,
         set_error("sep_eig_backward_batched: null handle");
         return SEP_ERR_ARG;
     }
-    SEP_HIP(hipSetDevice(h->device));
+    if (int rc = use_device(h)) return rc;
     return eig_backward_batched_f64(h, B, n, off_d, off_q, lam, Q, lam_bar, Q_bar, d_bar, e_bar);
 }
 
@@ -552,7 +562,7 @@ int sep_sort_eigen_f64_dev(sep_handle* h, int64_t n, const double* lam, const do
         set_error("sep_sort_eigen: null handle");
         return SEP_ERR_ARG;
     }
-    SEP_HIP(hipSetDevice(h->device));
+    if (int rc = use_device(h)) return rc;
     return sort_eigen_dev<double>(h, n, lam, Q, ldq, lam_out, Q_out, ldo, perm);
 }
 
@@ -563,7 +573,7 @@ int sep_sort_eigen_f32_dev(sep_handle* h, int64_t n, const float* lam, const flo
         set_error("sep_sort_eigen: null handle");
         return SEP_ERR_ARG;
     }
-    SEP_HIP(hipSetDevice(h->device));
+    if (int rc = use_device(h)) return rc;
     return sort_eigen_dev<float>(h, n, lam, Q, ldq, lam_out, Q_out, ldo, perm);
 }
 
@@ -574,7 +584,7 @@ int sep_sytrd_f64_dev(sep_handle* h, int64_t n, double* A, int64_t lda, double* 
         set_error("sep_sytrd: null handle");
         return SEP_ERR_ARG;
     }
-    SEP_HIP(hipSetDevice(h->device));
+    if (int rc = use_device(h)) return rc;
     return sytrd_f64(h, n, A, lda, d, e, tau);
 }
 
@@ -585,7 +595,7 @@ int sep_orgtr_f64_dev(sep_handle* h, int64_t n, const double* A, int64_t lda, co
         set_error("sep_orgtr: null handle");
         return SEP_ERR_ARG;
     }
-    SEP_HIP(hipSetDevice(h->device));
+    if (int rc = use_device(h)) return rc;
     return orgtr_f64(h, n, A, lda, tau, Q, ldq);
 }
 
