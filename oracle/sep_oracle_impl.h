@@ -454,7 +454,9 @@ int FN(orc_bulk_wave_parallel)(int64_t m, int64_t N, T *A, int64_t lda, const T 
  * reference does not offer but which leaves dv/ev identical).  Return value 0 on
  * success, -1 for odd n (the @assert at :155) unless allow_odd (the assert is the
  * reference's "limitation for now": nothing but its wave kernels' preconditions
- * depends on the parity, and apply modes 0, 2 and 3 have none); *converged / *iter as the
+ * depends on the parity, and apply modes 0, 2 and 3 have none), -2 for max_layers < 1,
+ * -3 when apply mode 1 meets a grid outside the wave preconditions, -4 for n < 1 (the
+ * reference cannot allocate its (n-1) x max_layers grid there); *converged / *iter as the
  * reference's return tuple; *nrot counts the rotations the chase created.
  * apply_mode: 0 = layer-major sequential grid apply, 1 = wave order (requires the
  * wave preconditions, k even and k <= n-2), 2 = chronological log (no grid),
@@ -465,6 +467,7 @@ int FN(orc_qr_algorithm)(int64_t n, T *dv, T *ev, T *Q, int64_t ldq, int64_t max
 {
     if (n % 2 != 0 && !allow_odd) return -1; /* :155 */
     if (max_layers < 1) return -2;
+    if (n < 1) return -4; /* :158 allocates an (n-1) x max_layers grid: no such array */
     int64_t start = 1, to = n, iter = 0;
     T *givens = (T *)malloc(sizeof(T) * 2 * (size_t)(n - 1) * (size_t)max_layers);
     int64_t *lt = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
