@@ -19,13 +19,18 @@
 //   k_trd_tiles   one 64 x 64 lower tile per workgroup: update, store, and the tile's
 //                 contributions A_IJ v_J (to row block I) and A_IJ^T v_I (to row block J)
 //                 written to a partial slot of their own (slot = the other tile index);
-//   k_trd_reduce  sums the slots in a fixed order (deterministic, no atomics);
-//   k_trd_vec     one workgroup: w of the previous column, its update applied to column i,
-//                 d[i], and the reflector of column i (dlarfg: beta = -sign(alpha)
-//                 hypot(alpha, |x|), tau = (beta - alpha) / beta, v = x / (alpha - beta)).
-// The explicit factor is accumulated backwards (Q = H(0) (H(1) (... H(n-2)))), again with
-// each reflector's rank-1 update applied in the pass that forms the next reflector's
-// column dots z = v^T Q.
+//   k_trd_reduce  sums the slots in a fixed order (deterministic, no atomics); up to 16
+//                 slots the next column step sums them itself;
+//   k_trd_vec_r   one workgroup, the column in registers (k_trd_vec: passes over memory
+//                 for columns beyond 8192 rows): w of the previous column, its update applied
+//                 to column i, d[i], and the reflector of column i (dlarfg: beta =
+//                 -sign(alpha) hypot(alpha, |x|), tau = (beta - alpha) / beta,
+//                 v = x / (alpha - beta)).
+// The explicit factor: below 256 rows accumulated backwards one reflector at a time
+// (Q = H(0) (H(1) (... H(n-2)))), each reflector's rank-1 update applied in the pass that
+// forms the next reflector's column dots z = v^T Q; from 256 rows in blocks of 64
+// reflectors (compact WY: Q -= V T V^T Q with library GEMMs, the products over the block's
+// rows split along those rows).
 #include <rocblas/rocblas.h>
 
 #include "sep_internal.hpp"
