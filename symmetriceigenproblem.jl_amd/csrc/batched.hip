@@ -516,7 +516,7 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
     // same buffers; the apply (apply stream) waits for the chase; the finished count is
     // copied to pinned host memory after the chase.  `first` marks the rounds whose
     // buffers no earlier round of the same sequence used.
-    auto enqueue_round = [&](int64_t r, bool first) -> hipError_t {
+    auto enqueue_chase = [&](int64_t r, bool first) -> hipError_t {
         const int b = (int)(r & 1);
         hipError_t e = hipSuccess;
         if (!first && (e = hipStreamWaitEvent(cs, applied[b], 0)) != hipSuccess) return e;
@@ -530,10 +530,16 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
                                cfg.ML, L.nmax, d_count, in_lds);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = hipMemcpyAsync(pin, d_count, sizeof(int), hipMemcpyDeviceToHost, cs)) !=
-                hipSuccess ||
-            (e = hipEventRecord(chased[b], cs)) != hipSuccess ||
-            (e = hipStreamWaitEvent(as, chased[b], 0)) != hipSuccess)
+            hipSuccess)
             return e;
+        return hipEventRecord(chased[b], cs);
+    };
+    // The apply of round r, after its chase (chase_done: that chase's event needs no wait,
+    // because it completed before this work was enqueued).
+    auto enqueue_apply = [&](int64_t r, bool chase_done) -> hipError_t {
+        const int b = (int)(r & 1);
+        hipError_t e = hipSuccess;
+        if (!chase_done && (e = hipStreamWaitEvent(as, chased[b], 0)) != hipSuccess) return e;
         if (!Q) {
             // eigenvalues only: nothing to apply (the ordering events stay the same)
         } else if (cfg.KF == 16) {
@@ -554,7 +560,8 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
     if (h->batch_eager) {
         // one round at a time; the host reads the finished count after each chase
         for (; round < max_rounds && done < B; ++round) {
-            if (enqueue_round(round, round < 2) != hipSuccess ||
+            if (enqueue_chase(round, round < 2) != hipSuccess ||
+                enqueue_apply(round, false) != hipSuccess ||
                 hipEventSynchronize(chased[round & 1]) != hipSuccess) {
                 rc = SEP_ERR_HIP;
                 break;
@@ -562,12 +569,19 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
             done = pin[0];
         }
     } else {
-        // kGraphRounds rounds captured once into a graph (both streams: the chase stream
-        // forks from and joins the apply stream inside the capture) and replayed until the
-        // finished count reaches B.  Finished problems return at once from the chase and
-        // leave nothing to apply, so rounds past the end cost two near-empty launches.
-        // Successive replays on the apply stream are ordered, which supplies the
-        // cross-replay dependency of the first two rounds of a replay.
+        // The chase of round 0 runs on its own; then kGraphRounds pairs (the apply of round
+        // r beside the chase of round r + 1) are captured once into a graph (the chase
+        // stream forks from and joins the apply stream inside the capture) and replayed
+        // until the finished count reaches B.  A replay therefore ends between two pairs,
+        // which are synchronised anyway (the chase of round r + 2 waits for the apply of
+        // round r, which used the same buffers), so no apply runs alone at a replay
+        // boundary, as it would if a replay held whole rounds (chase, then apply): replays
+        // on one stream run one after the other.  When the count reaches B after the chase
+        // of round c, no problem was left with rotations in round c, so the missing apply
+        // of the last replay's last chase has nothing to do.  Finished problems return at
+        // once from the chase and leave nothing to apply, so rounds past the end cost two
+        // near-empty launches.  Successive replays on the apply stream are ordered, which
+        // supplies the cross-replay dependencies of a replay's first pair.
         const std::vector<int64_t> key = {
             (int64_t)dv, (int64_t)ev, (int64_t)Q, (int64_t)d_probs, (int64_t)d_work,
             (int64_t)d_st, (int64_t)d_depth[0], (int64_t)d_lt[0], (int64_t)d_grid[0],
@@ -584,7 +598,14 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
             hipError_t e = hipStreamBeginCapture(as, hipStreamCaptureModeThreadLocal);
             if (e == hipSuccess) e = hipEventRecord(h->batch_ev[4], as);
             if (e == hipSuccess) e = hipStreamWaitEvent(cs, h->batch_ev[4], 0);
-            for (int r = 0; r < kGraphRounds && e == hipSuccess; ++r) e = enqueue_round(r, r < 2);
+            // pair j: the chase of round j + 1 (waiting for the apply of round j - 1 of this
+            // replay, if any) and the apply of round j (its chase ran in this replay's
+            // previous pair, or before the replay when j = 0); rounds relative to the
+            // replay's first apply, whose round is even
+            for (int j = 0; j < kGraphRounds && e == hipSuccess; ++j) {
+                e = enqueue_chase(j + 1, j == 0);
+                if (e == hipSuccess) e = enqueue_apply(j, j == 0);
+            }
             if (e == hipSuccess) e = hipEventRecord(h->batch_ev[5], cs);
             if (e == hipSuccess) e = hipStreamWaitEvent(as, h->batch_ev[5], 0);
             const hipError_t ee = hipStreamEndCapture(as, &graph);
@@ -598,10 +619,14 @@ int run_batched(sep_handle* h, int64_t B, const int64_t* n, const int64_t* off_d
             }
             h->batch_graph_key = key;
         }
-        // one replay in flight ahead of the one whose count the host reads
+        // the chase of round 0, then one replay in flight ahead of the one whose count the
+        // host reads
         hipEvent_t ran = h->batch_ev[6];  // end-of-replay marker (never captured)
-        if (hipGraphLaunch(h->batch_graph, as) != hipSuccess) rc = SEP_ERR_HIP;
-        round = kGraphRounds;
+        if (enqueue_chase(0, true) != hipSuccess ||
+            hipStreamWaitEvent(as, chased[0], 0) != hipSuccess ||
+            hipGraphLaunch(h->batch_graph, as) != hipSuccess)
+            rc = SEP_ERR_HIP;
+        round = 1 + kGraphRounds;  // chases enqueued so far
         while (rc == SEP_OK && round < max_rounds) {
             if (hipEventRecord(ran, as) != hipSuccess ||
                 hipGraphLaunch(h->batch_graph, as) != hipSuccess ||
