@@ -14,18 +14,19 @@
 //    row, loaded with ordinary vector loads one whole interval ahead, and every FMA takes
 //    its coefficient through the DPP64 `row_newbcast:j` source modifier, which broadcasts
 //    lane j of each row.
-//  * FP64 FMA issue rate depends on how many waves share a SIMD, not on the DPP operand:
-//    straight-line FMA streams (plain VGPR, SGPR or row_newbcast operand alike) reach
-//    ~66% of peak with one wave per SIMD, ~79% with two, ~87% with four and ~92% with
-//    eight; this kernel's three-shear pattern 75% / 86% with two / four.  Two 64-row
-//    geometries are built: "wide" (G = 8 waves of KG = 8 layers, S = 16 steps per barrier
-//    interval, two waves per SIMD) and "deep" (G = 16 waves of KG = 4 layers, S = 8, four
-//    waves per SIMD); wide is used for large matrices: deep synchronises twice as often
-//    and moves twice the LDS traffic per FMA, and the chip runs it at a lower clock (the
-//    apply is power-limited: ~1.85 GHz for deep against ~2.0 GHz for wide).  The first
-//    wave prefetches Q two or three intervals ahead.  Below 64 * 256 rows these leave CUs
-//    idle; the 16- and 32-row geometries further down (one layer group per DPP row,
-//    several workgroups per CU) take over.
+//  * FP64 FMA issue rate depends on how many waves share a SIMD, hardly on the operand
+//    source: register-only streams of the three-shear pattern (eight chains of three
+//    dependent FMAs) issue an FMA in ~63% of the cycles with one wave per SIMD, ~83%
+//    with two and ~93% with four (row_newbcast operand; a plain VGPR or SGPR coefficient
+//    and the VGPR bank of the operands change this by 1-3%), at 2.24-2.42 GHz on all
+//    CUs.  The 64-row geometry for large matrices is "wide": G = 8 waves of KG = 8
+//    layers, S = 16 steps per barrier interval, two waves per SIMD ("small", KG = 4 and
+//    S = 8, serves grids of at most 32 layers).  Four waves per SIMD would need 4 layers
+//    per wave, which synchronises twice as often and moves twice the LDS traffic per FMA;
+//    that geometry measured slower and is not built.  The first wave prefetches Q two or
+//    three intervals ahead.  Below 64 * 256 rows these leave CUs idle; the 16- and 32-row
+//    geometries further down (one layer group per DPP row, several workgroups per CU)
+//    take over.
 //  * Within an interval each wave lowers its priority as it progresses, so the younger of
 //    two waves sharing a SIMD is not starved until the older one waits at the barrier,
 //    and reads its LDS inputs and coefficient rows two steps ahead instead of all at the
@@ -1127,7 +1128,15 @@ enum ApplyGeomId { kGeomWide = 2, kGeomSmall = 5, kGeomR16Deep = 7, kGeomR32 = 9
 // drift apart: at m = 16384 r16's workgroups take 1.9 to 3.4 ms).  Also measured at 16384:
 // wide with Q prefetched three intervals ahead 2.57 ms; two passes of 32 layers with two
 // workgroups per CU 3.11 ms (KG 8, 4 waves) and 3.55 ms (KG 4, 8 waves); neighbour-to-
-// neighbour interval counters in LDS instead of the workgroup barrier 2.66 ms.  Float32,
+// neighbour interval counters in LDS instead of the workgroup barrier 2.66 ms.  (deep,
+// r16 and r32deep are no longer built.)  Where wide's time goes at 16384, from timing-only
+// copies with one cost removed each (2.44-2.50 ms as built): the middle waves' coefficient
+// rows not loaded 2.19-2.28 ms, or all coefficient rows read from LDS with no global loads
+// 2.44 ms (the cost is the 512 bytes per wave and step written into VGPRs, whichever way
+// they come); no Q loads or stores 2.08 ms; no barrier 2.32 ms, though its clock then
+// drops (1.88 GHz): the apply runs at the power limit, so cycles saved partly return as
+// a lower clock; no priority ramp 2.51 ms; no column-scale FMA in the first wave 2.44 ms.
+// Float32,
 // m = n, k = 64 (ms): 2000: wide 0.276, r16deep 0.164; 4096: 0.531, 0.309; 8192: 1.030,
 // 0.700; 16384: 2.038, 2.191 (r32 0.22 / 0.42 / 0.88 / 2.15).
 template <typename T>
