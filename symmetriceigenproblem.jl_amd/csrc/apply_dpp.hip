@@ -1110,13 +1110,40 @@ int run_r16(sep_handle* th, hipStream_t st, const ApplyMem<T>& mem, int64_t m, i
     return SEP_OK;
 }
 
-using G16Deep = G16<4, 4, 8>;   // 64 layers per pass: 4 waves of 4 groups of 4 layers
-using G32Wide = G16<8, 4, 16, 32>;  // 32-row blocks: 4 waves of 2 groups of 8 layers
+// 64 layers per pass: 4 waves of 4 groups of 4 layers, barrier intervals of 8, 16 or 24
+// steps.  Longer intervals synchronise less often and take more LDS per workgroup (40, 80,
+// 120 KB in Float64): they pay while the workgroups still fit the CUs at that size.
+using G16Deep = G16<4, 4, 8>;
+using G16Deep16 = G16<4, 4, 16>;
+using G16Deep24 = G16<4, 4, 24>;
 
-// The four geometries (SEP_APPLY names them wide, small, r16deep, r32).
-enum ApplyGeomId { kGeomWide = 2, kGeomSmall = 5, kGeomR16Deep = 7, kGeomR32 = 9 };
+// The five geometries (SEP_APPLY names them wide, small, r16deep, r16s16, r16s24).
+enum ApplyGeomId {
+    kGeomWide = 2,
+    kGeomSmall = 5,
+    kGeomR16Deep = 7,
+    kGeomR16S16 = 8,
+    kGeomR16S24 = 10
+};
 
-// Measured kernel time per apply, m = n, k = 64 (ms):
+// Interval length of the 16-row geometry, kernel time per apply, m = n, k = 64 (ms; one
+// box, outputs bit for bit the same; r32 is the 32-row geometry G16<8, 4, 16, 32>):
+//   Float64  S = 8   S = 16  S = 24  r32      Float32  S = 8   S = 16  S = 24  r32
+//   2000     0.191   0.168   0.167            2000     0.169   0.150   0.152   0.224
+//   3000     0.274   0.239   0.238
+//   4096     0.381   0.326   0.317            4096     0.321   0.279   0.273   0.420
+//   5000     0.530   0.497   0.690
+//   6144     0.708   0.633   0.832   0.698    6144     0.528   0.496   0.509   0.621
+//   8192     1.012   0.875   1.114   1.029    8192     0.726   0.680   0.688   0.863
+//   9000     1.252   1.379           1.484
+//   10240    1.408   1.630           1.674
+//   12000    1.709   1.870   2.297            12000    1.364   1.239   1.487   1.572
+//   14336                                     14336    1.863   1.826   2.150   1.878
+// (wide, Float32: 1.612 at 14336, 1.838 at 16384.)  A 16-row workgroup takes 40 / 80 /
+// 120 KB of LDS in Float64 (S = 8 / 16 / 24), half in Float32: S = 24 pays while the
+// workgroups fit one per CU, S = 16 while they fit two (Float64) or four (Float32) per CU.
+// r32 is no longer built.
+// Earlier measurements, kernel time per apply, m = n, k = 64 (ms):
 //            wide   deep   r16    r16deep r32    r32deep
 //   2000     0.304  0.317  0.200  0.192   0.227  0.259
 //   4096     0.589  0.611  0.383  0.373   0.429  0.494
@@ -1144,9 +1171,12 @@ int choose_geom(const sep_handle* h, int64_t m, int64_t k)
 {
     if (h && h->apply_geom) return h->apply_geom;  // SEP_APPLY (tests force each in turn)
     if (k <= 32) return kGeomSmall;
-    if (sizeof(T) == 4) return m <= 12288 ? kGeomR16Deep : kGeomWide;
-    if (m <= 6144 || (m > 10240 && m <= 14336)) return kGeomR16Deep;
-    if (m <= 10240) return kGeomR32;
+    const int64_t cus = h && h->num_cus > 0 ? h->num_cus : 256;
+    const int64_t blocks = (m + 15) / 16;  // 16-row workgroups
+    if (sizeof(T) == 4) return m <= 12288 ? kGeomR16S16 : kGeomWide;
+    if (blocks <= cus) return kGeomR16S24;
+    if (blocks <= 2 * cus) return kGeomR16S16;
+    if (m <= 14336) return kGeomR16Deep;
     return kGeomWide;
 }
 
@@ -1157,7 +1187,8 @@ size_t table_bytes(int geom, int64_t n)
         case kGeomWide: return sizeof(T) * DppDims<GeomWide>(n).table_elems();
         case kGeomSmall: return sizeof(T) * DppDims<GeomSmall>(n).table_elems();
         case kGeomR16Deep: return sizeof(T) * R16Dims<G16Deep>(n).table_elems();
-        default: return sizeof(T) * R16Dims<G32Wide>(n).table_elems();
+        case kGeomR16S16: return sizeof(T) * R16Dims<G16Deep16>(n).table_elems();
+        default: return sizeof(T) * R16Dims<G16Deep24>(n).table_elems();
     }
 }
 
@@ -1173,8 +1204,10 @@ int launch_geom(int geom, sep_handle* th, hipStream_t st, const ApplyMem<T>& mem
             return run_dpp<T, GeomSmall>(th, st, mem, m, n, A, lda, cs, ld_cs, k, rotations);
         case kGeomR16Deep:
             return run_r16<T, G16Deep>(th, st, mem, m, n, A, lda, cs, ld_cs, k, rotations);
+        case kGeomR16S16:
+            return run_r16<T, G16Deep16>(th, st, mem, m, n, A, lda, cs, ld_cs, k, rotations);
         default:
-            return run_r16<T, G32Wide>(th, st, mem, m, n, A, lda, cs, ld_cs, k, rotations);
+            return run_r16<T, G16Deep24>(th, st, mem, m, n, A, lda, cs, ld_cs, k, rotations);
     }
 }
 

@@ -259,7 +259,7 @@ int sep_create(sep_handle** out, int device)
     if (const char* ap = std::getenv("SEP_APPLY")) {
         // apply geometry (unset: chosen by the number of rows and layers)
         static const struct { const char* name; int geom; } kGeoms[] = {
-            {"wide", 2}, {"small", 5}, {"r16deep", 7}, {"r32", 9}};
+            {"wide", 2}, {"small", 5}, {"r16deep", 7}, {"r16s16", 8}, {"r16s24", 10}};
         for (const auto& g : kGeoms)
             if (std::strcmp(ap, g.name) == 0) h->apply_geom = g.geom;
     }

@@ -107,7 +107,7 @@ def test_full_size_16384_rows_subset(sep, handle, dtype):
     assert rel <= (1e-13 if dtype == np.float64 else 1e-4)
 
 
-GEOMETRIES = ["wide", "small", "r16deep", "r32"]
+GEOMETRIES = ["wide", "small", "r16deep", "r16s16", "r16s24"]
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])

@@ -29,8 +29,9 @@ def _inputs(sep, m, n, k, seed):
     return A, g
 
 
-# (m, n, k): the size rule picks wide (m > 14336), r32 (6144 < m <= 10240), r16deep
-# (m <= 6144) and small (k <= 32); each pair is a larger call then a smaller one
+# (m, n, k): on a 256-CU device the size rule picks wide (m > 14336), r16s16 (4096 < m <=
+# 8192), r16s24 (m <= 4096) and small (k <= 32); each pair is a larger call then a smaller
+# one
 PAIRS = [
     ((16384, 2000, 64), (15000, 700, 64)),
     ((8192, 16384, 64), (7000, 3000, 130)),
@@ -59,7 +60,7 @@ def test_noalloc_rejects_orders_beyond_its_scratch(sep):
     import torch
     h = sep.Handle(0)
     try:
-        m, n, k = 8000, 40000, 64  # the 32-row geometry needs well over 48 MiB here
+        m, n, k = 8000, 40000, 64  # the 16-row geometry (r16s16) needs well over 48 MiB here
         A = torch.full((n, m), 3.0, dtype=torch.float64, device="cuda").t()
         g = torch.zeros((k, n - 1, 2), dtype=torch.float64, device="cuda")
         rc = _call(sep, "sep_bulk_rmul_f64_dev_noalloc", h, A, g)
