@@ -244,7 +244,15 @@ __device__ __forceinline__ void apply_item(const BWork wk, const BProb* __restri
             const int q = lane + kLanes * e;
             const int s = q / KF, l = q % KF;
             const bool on = sin_[e] && L0 + l < slt[e];
-            const double p = on ? sv[e].y / (1.0 + sv[e].x) : 0.0;
+            // p = s / (1 + c) from the hardware reciprocal, one Newton step and one
+            // remainder correction (a rounding or two from the IEEE quotient: 1 + c lies in
+            // [1 - 1/sqrt 2, 2] for every rotation the chase produces); stale entries past a
+            // column's layer count may give anything here and are replaced by 0
+            const double den = 1.0 + sv[e].x;
+            double rc = __builtin_amdgcn_rcp(den);
+            rc = __builtin_fma(rc, __builtin_fma(-den, rc, 1.0), rc);
+            const double q0 = sv[e].y * rc;
+            const double p = on ? __builtin_fma(__builtin_fma(-den, q0, sv[e].y), rc, q0) : 0.0;
             const double qq = on ? -sv[e].y : 0.0;
             stc[buf][s][l / 8][2 * (l % 8)] = p;
             stc[buf][s][l / 8][2 * (l % 8) + 1] = qq;
@@ -273,21 +281,38 @@ __device__ __forceinline__ void apply_item(const BWork wk, const BProb* __restri
             stage_store(buf, L0);
             __syncthreads();
             if (b + 1 < nblk) stage_load(t0 + SB, L0);  // in flight during this stage
-            // byte offsets of the columns loaded / stored at step 0 of this stage; a
-            // negative column wraps to an offset beyond the range
-            const uint32_t ld0 = (uint32_t)(t0 + 1 + PA) * colb;
-            const uint32_t st0 = (uint32_t)(t0 - 2 * KF + 2) * colb;
+            // byte offsets of the columns loaded / stored at step 0 of this stage (a
+            // negative column wraps to an offset beyond the range), advanced by one column
+            // per step: the asm keeps them running vector adds, so that the compiler does not
+            // fold them into per-step scalar constants (which spill at this register count)
+            uint32_t lo[R], so[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                lo[r] = rowb[r] + (uint32_t)(t0 + 1 + PA) * colb;
+                so[r] = rowb[r] + (uint32_t)(t0 - 2 * KF + 2) * colb;
+            }
+            // the coefficients of step s are read from LDS one step ahead
+            double cfn[NCR];
+#pragma unroll
+            for (int h = 0; h < NCR; ++h) cfn[h] = stc[buf][0][h][lane & 15];
             Unroll<SB, 0>::run([&](auto sc) {
                 constexpr int s = decltype(sc)::value;
                 constexpr int k = s % PA;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     win[r][(s + 1) % W] = qn[r][k];  // column t + 1 enters the window
-                    qn[r][k] = buf_load(qr, rowb[r] + ld0 + (uint32_t)s * colb);
+                    qn[r][k] = buf_load(qr, lo[r]);
+                    lo[r] += colb;
+                    asm volatile("" : "+v"(lo[r]));
                 }
                 double cf[NCR];
 #pragma unroll
-                for (int h = 0; h < NCR; ++h) cf[h] = stc[buf][s][h][lane & 15];
+                for (int h = 0; h < NCR; ++h) {
+                    cf[h] = cfn[h];
+                    if constexpr (s + 1 < SB) cfn[h] = stc[buf][s + 1][h][lane & 15];
+                }
+                // the next step's reads stay ahead of this step's FMAs
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     if constexpr (KF == 4) {
@@ -301,8 +326,12 @@ __device__ __forceinline__ void apply_item(const BWork wk, const BProb* __restri
                     }
                 }
 #pragma unroll
-                for (int r = 0; r < R; ++r)  // column t - 2 KF + 2 leaves the window
-                    buf_store(qr, rowb[r] + st0 + (uint32_t)s * colb, win[r][(s + 2) % W]);
+                for (int r = 0; r < R; ++r) {  // column t - 2 KF + 2 leaves the window
+                    buf_store(qr, so[r], win[r][(s + 2) % W]);
+                    so[r] += colb;
+                    asm volatile("" : "+v"(so[r]));
+                }
+                __builtin_amdgcn_sched_barrier(0);
             });
         }
     }
