@@ -19,6 +19,10 @@
 // batches whose orders are all at most 512 use 64 x 64 block tiles (waves of 32 x 32).
 // Problems are described by a table, so the single and the batched entry points share the
 // kernels (grid z = problem; tiles beyond a problem's order exit at once).
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
 #include "sep_internal.hpp"
 
 namespace sep {
@@ -67,12 +71,18 @@ __device__ __forceinline__ void tile_of(int xcds, int& tx, int& ty)
 // CU and each SIMD has a second wave to issue MFMAs while one waits on memory or a barrier.
 // BT is the block tile: 128 (2 x 2 waves of 64 x 64) for large orders, 64 (2 x 2 waves of
 // 32 x 32) for batches of small problems, where a 128 tile would mostly compute padding.
+// One BT x BT tile (tx, ty) of problem pr: the product, then the epilogue of MODE.  As and
+// Bs are the workgroup's double-buffered LDS tiles; the caller separates two tiles run by
+// one workgroup with a barrier.
 template <int MODE, int BT>
-__global__ void __launch_bounds__(kThreads, 2)
-    k_bwd_gemm(const BwdProb* __restrict__ probs, const double* __restrict__ lam,
-               const double* __restrict__ lam_bar, const double* __restrict__ Q,
-               const double* __restrict__ Qb, double* __restrict__ M,
-               double* __restrict__ part, int64_t part_ld, int xcds)
+__device__ __forceinline__ void bwd_tile(const BwdProb pr, int tx, int ty,
+                                         const double* __restrict__ lam,
+                                         const double* __restrict__ lam_bar,
+                                         const double* __restrict__ Q,
+                                         const double* __restrict__ Qb, double* __restrict__ M,
+                                         double* __restrict__ part, int64_t part_ld,
+                                         double (&As)[2][kBK][BT + 4],
+                                         double (&Bs)[2][kBK][BT + 4])
 {
     constexpr int LDS_LD = BT + 4;
     constexpr int WT = BT / 32;          // 16 x 16 MFMA tiles per wave and dimension
@@ -80,13 +90,8 @@ __global__ void __launch_bounds__(kThreads, 2)
     constexpr int EPT = kBK / TPI;       // elements each of them loads
     constexpr int EPK = BT / 16;         // second product: elements per thread along i
     static_assert(EPT * TPI == kBK && EPK * 16 == BT, "tile geometry");
-    __shared__ double As[2][kBK][LDS_LD];
-    __shared__ double Bs[2][kBK][LDS_LD];
 
-    const BwdProb pr = probs[blockIdx.z];
     const int n = pr.n, ld = pr.ld;
-    int tx, ty;
-    tile_of(xcds, tx, ty);
     const int i0 = tx * BT, j0 = ty * BT;
     if (i0 >= n || j0 >= n) return;
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
@@ -243,6 +248,38 @@ __global__ void __launch_bounds__(kThreads, 2)
     }
 }
 
+// Grid mode (tiles == nullptr): one tile per workgroup, (x, y) the tile, z the problem.
+// List mode: the resident workgroups run the list's tiles blockIdx.x, + gridDim.x, ...
+// For batches of small problems most tiles are short (orders 64..256, a few k steps), and
+// one workgroup per tile left the CUs waiting on workgroup dispatch (0.5 waves resident per
+// SIMD on average, the MFMA pipe ~35% busy); resident workgroups that loop over the tiles
+// keep it fed.  Entries with prob < 0 pad the list's XCD layout.
+template <int MODE, int BT>
+__global__ void __launch_bounds__(kThreads, 2)
+    k_bwd_gemm(const BwdProb* __restrict__ probs, const double* __restrict__ lam,
+               const double* __restrict__ lam_bar, const double* __restrict__ Q,
+               const double* __restrict__ Qb, double* __restrict__ M,
+               double* __restrict__ part, int64_t part_ld, int xcds,
+               const int4* __restrict__ tiles, int ntiles)
+{
+    __shared__ double As[2][kBK][BT + 4];
+    __shared__ double Bs[2][kBK][BT + 4];
+    if (tiles) {
+        for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
+            const int4 tl = tiles[w];
+            if (tl.x < 0) continue;
+            bwd_tile<MODE, BT>(probs[tl.x], tl.y, tl.z, lam, lam_bar, Q, Qb, M, part, part_ld,
+                               As, Bs);
+            __syncthreads();  // this tile's last LDS reads before the next tile's stores
+        }
+    } else {
+        int tx, ty;
+        tile_of(xcds, tx, ty);
+        bwd_tile<MODE, BT>(probs[blockIdx.z], tx, ty, lam, lam_bar, Q, Qb, M, part, part_ld, As,
+                           Bs);
+    }
+}
+
 // d_bar_k = sum_ct S0[ct][k];  e_bar_k = sum_ct (S1[ct][k] + S2[ct][k+1])
 __global__ void __launch_bounds__(256)
     k_bwd_reduce(const BwdProb* __restrict__ probs, const double* __restrict__ part,
@@ -276,38 +313,78 @@ int run_backward(sep_handle* h, const std::vector<BwdProb>& probs, int nmax, int
     // 64-tiles when every order is small (the 128 tile's padding would dominate)
     const int bt = nmax <= kSmallTileMaxN ? 64 : kBT;
     const int nct = (nmax + bt - 1) / bt;
-    const size_t meta_bytes = sizeof(BwdProb) * (size_t)B;
+    // Batches run a tile list (list mode of k_bwd_gemm): every problem's tiles, problems
+    // largest first, each problem on one XCD (the XCD with the least work so far, in n^3),
+    // so that its tiles share Q and Q_bar in that XCD's L2.  Workgroup g runs the list
+    // entries g, g + G, ... on XCD g mod X (G a multiple of X), so XCD x's entries sit at
+    // positions x, x + X, ...; shorter queues are padded with prob = -1.
+    std::vector<int4> tiles;
+    if (B > 1) {
+        const int X = std::max(1, h->num_xcds);
+        std::vector<int> order((size_t)B);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return probs[a].n > probs[b].n; });
+        std::vector<std::vector<int4>> q((size_t)X);
+        std::vector<double> load((size_t)X, 0.0);
+        for (int p : order) {
+            const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            const int nt = (probs[p].n + bt - 1) / bt;
+            for (int ty = 0; ty < nt; ++ty)
+                for (int tx = 0; tx < nt; ++tx) q[x].push_back(make_int4(p, tx, ty, 0));
+            load[x] += (double)probs[p].n * probs[p].n * probs[p].n;
+        }
+        size_t len = 0;
+        for (auto& v : q) len = std::max(len, v.size());
+        tiles.assign(len * X, make_int4(-1, 0, 0, 0));
+        for (int x = 0; x < X; ++x)
+            for (size_t j = 0; j < q[x].size(); ++j) tiles[j * X + x] = q[x][j];
+    }
+    const size_t probs_bytes = (sizeof(BwdProb) * (size_t)B + 15) / 16 * 16;
+    const size_t meta_bytes = probs_bytes + sizeof(int4) * tiles.size();
     if (int rc = h->bwd_meta.reserve(meta_bytes)) return rc;
     if (int rc = h->bwd_M.reserve(sizeof(double) * (size_t)tot_m)) return rc;
     if (int rc = h->bwd_part.reserve(sizeof(double) * 3 * (size_t)nct * (size_t)(tot_d + 1)))
         return rc;
-    // the table is uploaded from a handle-owned host copy, and only when it changes
-    const char* raw = reinterpret_cast<const char*>(probs.data());
-    if (h->bwd_meta_host.size() != meta_bytes ||
-        std::memcmp(h->bwd_meta_host.data(), raw, meta_bytes) != 0 || h->bwd_meta_dev != h->bwd_meta.p) {
-        h->bwd_meta_host.assign(raw, raw + meta_bytes);
+    // the table and the tile list are uploaded from a handle-owned host copy, and only when
+    // they change
+    std::vector<char> raw(meta_bytes, 0);
+    std::memcpy(raw.data(), probs.data(), sizeof(BwdProb) * (size_t)B);
+    if (!tiles.empty())
+        std::memcpy(raw.data() + probs_bytes, tiles.data(), sizeof(int4) * tiles.size());
+    if (h->bwd_meta_host != raw || h->bwd_meta_dev != h->bwd_meta.p) {
+        h->bwd_meta_host = std::move(raw);
         SEP_HIP(hipMemcpyAsync(h->bwd_meta.p, h->bwd_meta_host.data(), meta_bytes,
                                hipMemcpyHostToDevice, h->stream));
         SEP_HIP(hipStreamSynchronize(h->stream));
         h->bwd_meta_dev = h->bwd_meta.p;
     }
     const BwdProb* dp = h->bwd_meta.as<BwdProb>();
+    const int4* dtiles =
+        tiles.empty() ? nullptr
+                      : reinterpret_cast<const int4*>(static_cast<char*>(h->bwd_meta.p) + probs_bytes);
+    const int ntiles = (int)tiles.size();
     const int64_t part_ld = tot_d + 1;
     double* Mb = h->bwd_M.as<double>();
     double* part = h->bwd_part.as<double>();
-    const dim3 grid(nct, nct, (unsigned)B);
+    // grid mode: (x, y) tiles, z problems; list mode: as many workgroups as fit the CUs at
+    // once (64-tiles: 35 KB of LDS and at most 128 registers per lane, four per CU; 128-tiles
+    // two)
+    const int64_t per_cu = bt == 64 ? 4 : 2;
+    const dim3 grid = dtiles ? dim3((unsigned)std::min<int64_t>(ntiles, per_cu * h->num_cus))
+                             : dim3(nct, nct, (unsigned)B);
     if (bt == 64) {
         hipLaunchKernelGGL((k_bwd_gemm<kModeM, 64>), grid, dim3(kThreads), 0, h->stream, dp,
-                           lam, lam_bar, Q, Q_bar, Mb, part, part_ld, h->num_xcds);
+                           lam, lam_bar, Q, Q_bar, Mb, part, part_ld, h->num_xcds, dtiles, ntiles);
         SEP_CHECK_LAUNCH();
         hipLaunchKernelGGL((k_bwd_gemm<kModeP, 64>), grid, dim3(kThreads), 0, h->stream, dp,
-                           lam, lam_bar, Q, Q_bar, Mb, part, part_ld, h->num_xcds);
+                           lam, lam_bar, Q, Q_bar, Mb, part, part_ld, h->num_xcds, dtiles, ntiles);
     } else {
         hipLaunchKernelGGL((k_bwd_gemm<kModeM, kBT>), grid, dim3(kThreads), 0, h->stream, dp,
-                           lam, lam_bar, Q, Q_bar, Mb, part, part_ld, h->num_xcds);
+                           lam, lam_bar, Q, Q_bar, Mb, part, part_ld, h->num_xcds, dtiles, ntiles);
         SEP_CHECK_LAUNCH();
         hipLaunchKernelGGL((k_bwd_gemm<kModeP, kBT>), grid, dim3(kThreads), 0, h->stream, dp,
-                           lam, lam_bar, Q, Q_bar, Mb, part, part_ld, h->num_xcds);
+                           lam, lam_bar, Q, Q_bar, Mb, part, part_ld, h->num_xcds, dtiles, ntiles);
     }
     SEP_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_bwd_reduce, dim3((nmax + 255) / 256, (unsigned)B), dim3(256), 0,
