@@ -23,7 +23,7 @@ __global__ void __launch_bounds__(256) k_fill_identity(T* __restrict__ grid, int
 }
 
 // Problems up to this order keep dv, ev and layer_thickness in LDS for the whole block
-// (20 bytes per index in Float64: 120 KiB beside the chase's 6 KiB of chunk staging).
+// (20 bytes per index in Float64: 120 KiB beside the chase's 24.6 KiB of chunk staging).
 // The chase's own memory traffic is a chain of dependent round trips -- the deflation scan
 // from `to` downwards, the thickness maximum, the shift's and the first rotation's
 // operands, each chunk's inputs -- which take ~0.1 us each from LDS against ~0.7 us from
@@ -52,7 +52,7 @@ __global__ void __launch_bounds__(64) k_chase_block(T* dv, T* ev, int* lt, T* gr
         // layer_thickness lives only within a block (chase_block resets it), so the LDS
         // copy needs neither loading nor writing back
         chase::Problem<T> P{s_dv, s_ev, s_lt, grid, n, max_layers, (int64_t)(n - 1)};
-        chase::chase_block<T, true, kFast>(P, tol, maxiter, st, lane);
+        chase::chase_block<T, true, kFast, true>(P, tol, maxiter, st, lane);
         __syncthreads();
         for (int q = lane; q < n; q += 64) dv[q] = s_dv[q];
         for (int q = lane; q < n - 1; q += 64) ev[q] = s_ev[q];
@@ -93,12 +93,12 @@ int fill_identity(hipStream_t s, T* grid, int64_t pairs)
 
 }  // namespace
 
-// The solve's pinned state words and its three events, created on the handle's first
-// solve and kept until sep_destroy.
+// The solve's pinned state words (two sets) and its four events, created on the handle's
+// first solve and kept until sep_destroy.
 static int solve_resources(sep_handle* h)
 {
     if (!h->solve_pinned)
-        SEP_HIP(hipHostMalloc((void**)&h->solve_pinned, sizeof(int64_t) * chase::ST_WORDS,
+        SEP_HIP(hipHostMalloc((void**)&h->solve_pinned, sizeof(int64_t) * 2 * chase::ST_WORDS,
                               hipHostMallocDefault));
     for (auto& e : h->solve_ev)
         if (!e) SEP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -144,41 +144,49 @@ int qr_algorithm_dev(sep_handle* h, int64_t n64, T* dv, T* ev, T* Q, int64_t ldq
     SEP_HIP(hipMemcpyAsync(dst, hst, sizeof(init), hipMemcpyHostToDevice, cs));
 
     hipEvent_t applied[2] = {h->solve_ev[0], h->solve_ev[1]};
-    hipEvent_t chased = h->solve_ev[2];
+    hipEvent_t chased[2] = {h->solve_ev[2], h->solve_ev[3]};
     bool applied_pending[2] = {false, false};
+    // the fast rotation is a Float64 option (Float32 chases always take the exact one)
+    const bool fast = std::is_same<T, double>::value && h->chase_fast;
 
-    int rc = SEP_OK;
-    int64_t total_rot = 0, iter = 0;
-    int conv = 1;
-    int cur = 0;
-    for (;;) {
-        T* grid = h->grid[cur].as<T>();
-        if (applied_pending[cur]) {
-            // the apply of two blocks ago still reads this grid buffer
-            if (hipStreamWaitEvent(cs, applied[cur], 0) != hipSuccess) { rc = SEP_ERR_HIP; break; }
-        }
-        if ((rc = fill_identity<T>(cs, grid, pairs))) break;
-        // the fast rotation is a Float64 option (Float32 chases always take the exact one)
-        if (std::is_same<T, double>::value && h->chase_fast)
+    // Block b chases into grid[b % 2] and copies its state words to hst[b % 2].  It starts
+    // from the state words block b - 1 left on the device, so the host enqueues block b + 1
+    // before it waits for block b's words: the chase stream runs block after block without
+    // a host round trip between them (n = 2000: 19 blocks, whose gaps fell from 148 to
+    // 26 us on average, 2.7 to 0.5 ms per solve, in kernel traces).  The block
+    // enqueued after the last one changes nothing: after a block that found nothing to
+    // chase, `to` is at the start and the next one stops at once; after one that stopped
+    // at maxiter, the next one stops before its first scan.
+    auto enqueue_block = [&](int slot) -> int {
+        T* grid = h->grid[slot].as<T>();
+        if (applied_pending[slot])  // the apply of two blocks ago still reads this grid
+            SEP_HIP(hipStreamWaitEvent(cs, applied[slot], 0));
+        if (int rc = fill_identity<T>(cs, grid, pairs)) return rc;
+        if (fast)
             launch_chase<T, true>(cs, dv, ev, h->lt.as<int>(), grid, n, (int)max_layers, tol,
                                   maxiter, dst);
         else
             launch_chase<T, false>(cs, dv, ev, h->lt.as<int>(), grid, n, (int)max_layers, tol,
                                    maxiter, dst);
-        if (hipGetLastError() != hipSuccess) {
-            set_error("chase launch failed");
-            rc = SEP_ERR_HIP;
-            break;
-        }
-        if (hipMemcpyAsync(hst, dst, sizeof(int64_t) * chase::ST_WORDS, hipMemcpyDeviceToHost,
-                           cs) != hipSuccess ||
-            hipEventRecord(chased, cs) != hipSuccess ||
-            hipEventSynchronize(chased) != hipSuccess) {
+        SEP_CHECK_LAUNCH();
+        SEP_HIP(hipMemcpyAsync(hst + slot * chase::ST_WORDS, dst,
+                               sizeof(int64_t) * chase::ST_WORDS, hipMemcpyDeviceToHost, cs));
+        SEP_HIP(hipEventRecord(chased[slot], cs));
+        return SEP_OK;
+    };
+
+    int64_t total_rot = 0, iter = 0;
+    int conv = 1;
+    int cur = 0;
+    int rc = enqueue_block(0);
+    while (rc == SEP_OK) {
+        if ((rc = enqueue_block(cur ^ 1))) break;
+        if (hipEventSynchronize(chased[cur]) != hipSuccess) {
             set_error("chase: state read-back failed: %s", hipGetErrorString(hipGetLastError()));
             rc = SEP_ERR_HIP;
             break;
         }
-        const int64_t* st = hst;
+        const int64_t* st = hst + cur * chase::ST_WORDS;
         iter = st[chase::ST_ITER];
         if (st[chase::ST_EARLY]) {  // :180-182 -- return without applying this block
             conv = 0;
@@ -189,20 +197,21 @@ int qr_algorithm_dev(sep_handle* h, int64_t n64, T* dv, T* ev, T* Q, int64_t ldq
         if (Q) {
             const int64_t ilo = st[chase::ST_ILO], ihi = st[chase::ST_IHI];
             const int64_t depth = st[chase::ST_MAXLAYER];
-            if (hipStreamWaitEvent(as, chased, 0) != hipSuccess) { rc = SEP_ERR_HIP; break; }
+            if (hipStreamWaitEvent(as, chased[cur], 0) != hipSuccess) { rc = SEP_ERR_HIP; break; }
             // rotations ilo..ihi (1-based) touch columns ilo-1 .. ihi (0-based)
             rc = bulk_apply_dev<T>(h, as, h->apply_work[cur], n, ihi - ilo + 2,
-                                   Q + (ilo - 1) * ldq, ldq, grid + 2 * (ilo - 1), n - 1, depth,
-                                   /*rotations=*/true);
+                                   Q + (ilo - 1) * ldq, ldq, h->grid[cur].as<T>() + 2 * (ilo - 1),
+                                   n - 1, depth, /*rotations=*/true);
             if (rc) break;
             if (hipEventRecord(applied[cur], as) != hipSuccess) { rc = SEP_ERR_HIP; break; }
             applied_pending[cur] = true;
         }
         cur ^= 1;
     }
-    // the caller's stream (apply stream) must also see the final dv/ev
-    (void)hipEventRecord(chased, cs);
-    (void)hipStreamWaitEvent(as, chased, 0);
+    // the caller's stream (apply stream) must also see the final dv/ev (and the block
+    // enqueued after the last one has finished with them)
+    (void)hipEventRecord(chased[0], cs);
+    (void)hipStreamWaitEvent(as, chased[0], 0);
     hipError_t se = hipStreamSynchronize(as);
     if (se != hipSuccess && rc == SEP_OK) {
         set_error("qr_algorithm: %s", hipGetErrorString(se));

@@ -288,8 +288,9 @@ __device__ __forceinline__ Rot<double> givens<double>(double f, double g)
 
 // The same rotation with its range test deferred: no scalar round trip and no branch per
 // step (a single chase wave waits on the vector-to-scalar transfer of each step's operands
-// and on the branch: with the deferred test the step takes ~141 ns instead of ~182 ns).  acc collects 2 (hi(|x|) - hi(2^-380) - 1) mod 2^32 of both operands (the offset
-// of the test above, doubled so that the shift drops the sign bit); a chunk of steps stayed
+// and on the branch: with the deferred test the step takes ~141 ns instead of ~182 ns).
+// acc collects 2 (hi(|x|) - hi(2^-380) - 1) mod 2^32 of both operands (the offset of the
+// test above, doubled so that the shift drops the sign bit); a chunk of steps stayed
 // in the common range iff acc < kRareAcc at its end, and is otherwise run again with
 // givens().  The sign rule is evaluated per lane (the values are the same in every lane).
 constexpr uint32_t kRareAcc = 2u * (kRangeHi - (kRangeLo + 1));
@@ -406,16 +407,25 @@ __device__ __forceinline__ int64_t grid_pair(const Problem<T>& P, int i, int lay
 // 64-entry groups whose loads are in flight together in the deflation scan and the
 // thickness maximum.  The scan runs down from `to` until it finds a negligible ev, which
 // in a single large problem is usually far away (often the whole active range), and the
-// maximum always reads its whole range, so each is a chain of dependent memory round trips:
-// a single solve takes wide rounds (a sweep of an n = 2000 problem spends ~4 round trips
-// on them instead of ~20); the batched chase, whose problems are at most a few groups
-// long and whose waves share their SIMDs, keeps narrow ones.
-template <bool kSolo>
+// maximum always reads its whole range, so each is a chain of dependent memory round trips.
+// A single solve whose problem sits in global memory (kLds false) takes wide rounds (a
+// sweep of an n = 8192 problem spends a few round trips on them instead of dozens).  With
+// the problem in LDS a round trip is short and a wide round mostly issues loads past the
+// deflation point that the scan never needed: narrow rounds are faster there.  The batched
+// chase, whose problems are at most a few groups long and whose waves share their SIMDs,
+// keeps narrow ones too.
+template <bool kSolo, bool kLds>
 struct ScanBatch {
-    static constexpr int scan = kSolo ? 8 : 2;
-    static constexpr int max = kSolo ? 32 : 8;
+    static constexpr int scan = kSolo && !kLds ? 8 : 2;
+    static constexpr int max = kSolo && !kLds ? 32 : 8;
 };
-constexpr int kStepUnroll = 8;  // chase steps per iteration of the chunk loop
+// chase steps per iteration of the chunk loop: one compare and taken branch per this many
+// steps.  A single solve runs 512-step chunks and unrolls 32 steps; the batched chase,
+// whose chunks are 64 steps, keeps 8 (its code size stays small beside the apply's).
+template <bool kSolo>
+struct StepUnroll {
+    static constexpr int n = kSolo ? 32 : 8;
+};
 // static LDS of the batched chase (single_shift's chunk staging, 64-step chunks), per wave
 constexpr unsigned kChaseLdsBytes = ((kWave + 2) * 2 + kWave * 4) * sizeof(double);
 
@@ -471,19 +481,24 @@ __device__ int max_thickness(const Problem<T>& P, int from, int to, int lane)
 // memory traffic of steps i0 + j, i0 + 64 + j, ... of each chunk).
 // kSolo: the chase wave has its SIMD to itself (single solves), so the step is bound by
 // the wave's own issue: the rotation's range test is deferred to the end of the chunk, and
-// chunks are 128 steps long.  Otherwise (the batched chase, several waves and the apply
-// per SIMD) the range test stays on the scalar unit per step (80-81.5 ms against 85 ms per
-// 4096-problem batch with the deferred test) and chunks are 64 steps, which keeps the
-// chunk staging at 3 KB of LDS per wave.  kFast (Float64): the fast rotation and the
-// dsteqr-style 2x2 update, with the deferred range test in both forms.
+// chunks are 512 steps long: fewer chunk ends, whose stores, range test and grid writes
+// stall the wave.  With the 32-step loop unroll and the narrow scan rounds of an
+// LDS-resident problem, an n = 2000 solve takes 259-263 ms against 266-269 ms with 128-step
+// chunks, 8-step unrolls and wide rounds, bit for bit the same.  Otherwise (the batched
+// chase, several waves and the apply per SIMD) the range test stays on the scalar unit per
+// step (80-81.5 ms against 85 ms per 4096-problem batch with the deferred test) and chunks
+// are 64 steps, which keeps the chunk staging at 3 KB of LDS per wave.  kFast (Float64):
+// the fast rotation and the dsteqr-style 2x2 update, with the deferred range test in both
+// forms.
 template <typename T, bool kSolo = true, bool kFast = false>
 __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int lane)
 {
     // kSolo: chunks of R 64 steps (lane j handles the memory traffic of steps j, j + 64, ...);
     // step j's (dv[i+1], ev[i+1]) at solo_in[j] (two pad entries for the read-ahead) and its
-    // outputs (nrm, B11'', c, s) at solo_out[j].  The batched chase keeps 64-step chunks
-    // (its outputs are deposited in registers, one step per lane).
-    constexpr int R = kSolo ? 2 : 1;
+    // outputs (nrm, B11'', c, s) at solo_out[j]: 24.6 KB of LDS for double, beside the
+    // problem's own 20 bytes per row when it sits in LDS (120 KB at n = 6144).  The batched
+    // chase keeps 64-step chunks (its outputs are deposited in registers, one step per lane).
+    constexpr int R = kSolo ? 8 : 1;
     constexpr int kChunk = R * kWave;
     __shared__ T solo_in[kChunk + 2][2];
     __shared__ T solo_out[kChunk][4];
@@ -626,13 +641,14 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                     bulge = s * B23;
                     e_cur = c * B23;
                 };
-                // kStepUnroll steps per loop iteration: one compare and taken branch per
-                // kStepUnroll steps (a taken branch costs a single wave more than its issue
-                // slot: per step 141 ns at 1, 129 at 4, 126 at 8, 125.6 at 16)
+                // kU steps per loop iteration: one compare and taken branch per kU steps (a
+                // taken branch costs a single wave more than its issue slot: per step 141 ns
+                // at 1, 129 at 4, 126 at 8, 125.6 at 16)
+                constexpr int kU = StepUnroll<kSolo>::n;
                 int j = 0;
-                for (; j + kStepUnroll <= cnt; j += kStepUnroll) {
+                for (; j + kU <= cnt; j += kU) {
 #pragma unroll
-                    for (int u = 0; u < kStepUnroll; ++u) step(j + u);
+                    for (int u = 0; u < kU; ++u) step(j + u);
                 }
                 for (; j < cnt; ++j) step(j);
             };
@@ -706,8 +722,9 @@ enum StateWord {
 };
 
 // One outer iteration of qr_algorithm! without the apply (src/bulge_chasing.jl:166-276):
-// layer_thickness is reset here; the grid must hold identity rotations on entry.
-template <typename T, bool kSolo = true, bool kFast = false>
+// layer_thickness is reset here; the grid must hold identity rotations on entry.  kLds:
+// P's dv, ev and lt point into LDS.
+template <typename T, bool kSolo = true, bool kFast = false, bool kLds = false>
 __device__ void chase_block(const Problem<T>& P, T tol, int64_t maxiter, int64_t* st, int lane)
 {
     const int start = 1;
@@ -727,7 +744,7 @@ __device__ void chase_block(const Problem<T>& P, T tol, int64_t maxiter, int64_t
         int from = to;
         if (from > start)
             from = __builtin_amdgcn_readfirstlane(
-                deflation_scan<T, ScanBatch<kSolo>::scan>(P, to, start, tol, lane) + 1);
+                deflation_scan<T, ScanBatch<kSolo, kLds>::scan>(P, to, start, tol, lane) + 1);
         if (to <= start) break;
         if (from == to) {
             if (lane == 0) P.ev[from - 2] = T(0);
@@ -736,7 +753,7 @@ __device__ void chase_block(const Problem<T>& P, T tol, int64_t maxiter, int64_t
             to -= 1;
         } else {
             const int thick = __builtin_amdgcn_readfirstlane(
-                max_thickness<T, ScanBatch<kSolo>::max>(P, from, to, lane));
+                max_thickness<T, ScanBatch<kSolo, kLds>::max>(P, from, to, lane));
             if (thick == P.max_layers) {
                 to = from - 1;
             } else {
@@ -754,7 +771,7 @@ __device__ void chase_block(const Problem<T>& P, T tol, int64_t maxiter, int64_t
         if (to <= start) break;
     }
     int maxlayer = 0;
-    if (did) maxlayer = max_thickness<T, ScanBatch<kSolo>::max>(P, 1, P.n, lane);
+    if (did) maxlayer = max_thickness<T, ScanBatch<kSolo, kLds>::max>(P, 1, P.n, lane);
     if (lane == 0) {
         st[ST_TO] = lbea;
         st[ST_LBEA] = lbea;

@@ -83,9 +83,10 @@ struct sep_handle {
     hipStream_t chase_stream = nullptr;  // chase stream (overlap with apply)
     bool own_stream = true;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
-    // single solves: pinned copy of the chase's state words, events applied[2] and chased
+    // single solves: pinned copies of the chase's state words (one per grid buffer), events
+    // applied[2] and chased[2]
     int64_t* solve_pinned = nullptr;
-    hipEvent_t solve_ev[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t solve_ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // scratch
     sep::DevBuf host_io_A, host_io_cs, host_io_d, host_io_e;
     sep::ApplyWork apply_work[2];
