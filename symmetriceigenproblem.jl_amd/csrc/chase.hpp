@@ -314,7 +314,7 @@ __device__ __forceinline__ Rot<double> givens<double>(double f, double g)
 // test above, doubled so that the shift drops the sign bit); a chunk of steps stayed
 // in the common range iff acc < kRareAcc at its end, and is otherwise run again with
 // givens().  The sign rule is one compare into a lane mask and two selects (SelSign).
-// Both forms cut the exact step from 51 to 48 instructions: the n = 2000 solve takes
+// Both forms cut the exact step from 51 to 49 instructions: the n = 2000 solve takes
 // 0.251-0.255 s against 0.258-0.262 s with the five-instruction sign rule and the
 // compiler's max tree, bit for bit the same.
 constexpr uint32_t kRareAcc = 2u * (kRangeHi - (kRangeLo + 1));
