@@ -500,6 +500,17 @@ __device__ int max_thickness(const Problem<T>& P, int from, int to, int lane)
     return wave_max(v);
 }
 
+// 32-bit LDS address of a __shared__ object, and back.
+__device__ __forceinline__ uint32_t lds_addr(const void* p)
+{
+    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(3))) T* lds_ptr(uint32_t a)
+{
+    return (__attribute__((address_space(3))) T*)(uintptr_t)a;
+}
+
 // single_shift!(H, from, to, mu, givens, layer_thickness), src/bulge_chasing.jl:11-144,
 // executed by the whole wave (all lanes compute the uniform recurrence; lane j owns the
 // memory traffic of steps i0 + j, i0 + 64 + j, ... of each chunk).
@@ -524,8 +535,11 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
     // chase keeps 64-step chunks (its outputs are deposited in registers, one step per lane).
     constexpr int R = kSolo ? 8 : 1;
     constexpr int kChunk = R * kWave;
-    __shared__ T solo_in[kChunk + 2][2];
-    __shared__ T solo_out[kChunk][4];
+    __shared__ __align__(16) T solo_in[kChunk + 2][2];
+    __shared__ __align__(16) T solo_out[kChunk][4];
+    // (x, y) pairs of the staging rows, as one 2 x sizeof(T) LDS access each
+    using V2 = T __attribute__((ext_vector_type(2)));
+    using LdsV2 = __attribute__((address_space(3))) V2;
     T* __restrict__ dv = P.dv;
     T* __restrict__ ev = P.ev;
     int* __restrict__ lt = P.lt;
@@ -625,12 +639,14 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
             auto run = [&](auto deferred) {
                 // the inputs of steps j + 1 and j + 2, read from LDS ahead of use
                 T a1D = l1D, a1E = l1E, a2D = l2D, a2E = l2E;
-                auto step = [&](int j) {
+                // in2 = solo_in[j + 2] and out = solo_out[j] for step j, as pairs
+                auto step = [&](const LdsV2* __restrict__ in2, LdsV2* __restrict__ out) {
                     const T B22 = a1D, B23 = a1E;
                     a1D = a2D;
                     a1E = a2E;
-                    a2D = solo_in[j + 2][0];
-                    a2E = solo_in[j + 2][1];
+                    const V2 nx = *in2;
+                    a2D = nx.x;
+                    a2E = nx.y;
                     const Rot<T> G = step_rotation(deferred, p1, bulge, acc);
                     const T c = G.c, s = G.s;
                     const T B11 = d_cur, B21 = e_cur;
@@ -654,10 +670,8 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                         B22pp = __builtin_fma(B21p, -s, B22p * c);
                     }
                     // every lane writes the same values (the LDS pipe, not the VALU)
-                    solo_out[j][0] = G.r;
-                    solo_out[j][1] = B11pp;
-                    solo_out[j][2] = c;
-                    solo_out[j][3] = s;
+                    out[0] = V2{G.r, B11pp};
+                    out[1] = V2{c, s};
                     p1 = B12pp;
                     d_cur = B22pp;
                     // B23 = ev[i+1]; the last step of the sweep has none: its prefetch slot
@@ -667,14 +681,27 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                 };
                 // kU steps per loop iteration: one compare and taken branch per kU steps (a
                 // taken branch costs a single wave more than its issue slot: per step 141 ns
-                // at 1, 129 at 4, 126 at 8, 125.6 at 16)
+                // at 1, 129 at 4, 126 at 8, 125.6 at 16).  The iteration's two LDS base
+                // addresses are pinned in VGPRs, each step reaching its slots by immediate
+                // offsets: the compiler otherwise copies the scalar base into a VGPR before
+                // every step's read and writes, two vector moves per step in the batched
+                // chase (46 -> 44 vector instructions; the 4096-problem batch stays within
+                // its 54-56 ms spread).  Not in the single solve's fast chase, whose reads
+                // the compiler then keeps behind the previous steps' writes (a wait per
+                // step instead of two per 32 steps).
                 constexpr int kU = StepUnroll<kSolo>::n;
                 int j = 0;
                 for (; j + kU <= cnt; j += kU) {
+                    uint32_t ib = lds_addr(&solo_in[j + 2][0]), ob = lds_addr(&solo_out[j][0]);
+                    if constexpr (!kSolo || !kFast) asm volatile("" : "+v"(ib), "+v"(ob));
+                    const LdsV2* in2 = lds_ptr<V2>(ib);
+                    LdsV2* out = lds_ptr<V2>(ob);
 #pragma unroll
-                    for (int u = 0; u < kU; ++u) step(j + u);
+                    for (int u = 0; u < kU; ++u) step(in2 + u, out + 2 * u);
                 }
-                for (; j < cnt; ++j) step(j);
+                for (; j < cnt; ++j)
+                    step(lds_ptr<V2>(lds_addr(&solo_in[j + 2][0])),
+                         lds_ptr<V2>(lds_addr(&solo_out[j][0])));
             };
             if constexpr (std::is_same<T, double>::value && (kSolo || kFast)) {
                 if constexpr (kFast)
