@@ -135,6 +135,40 @@ def test_not_converged_matches_oracle(sep, handle):
     assert np.linalg.norm(gQ - oQ) <= 1e-13 * n
 
 
+@pytest.mark.parametrize("maxiter", [2, 3, 4, 9, 24, 41])
+def test_not_converged_bitwise_in_any_block(sep, handle, maxiter):
+    """A solve stopped at maxiter leaves dv and ev as the oracle does, bit for bit, and
+    returns without applying the stopping block (src/bulge_chasing.jl:180-182), whichever
+    block it stops in (max_layers = 4 keeps the blocks short, so these stops fall in
+    different blocks).  The host enqueues each chase block before reading the previous
+    one's state, so a block also runs after the stopping one: it must change nothing."""
+    n = 120
+    d, e = sep.inputs.tridiag_linear(n, seed=11)
+    (od, oe, oQ, oconv, oit, _), (gd, ge, gQ, conv, it, _) = _solve_both(
+        sep, handle, d, e, 4, maxiter=maxiter)
+    assert not oconv and not conv and it == oit == maxiter + 1
+    assert np.array_equal(gd, od) and np.array_equal(ge, oe)
+    assert np.linalg.norm(gQ - oQ) <= 1e-13 * n
+
+
+def test_solve_after_solve_on_one_handle(sep, handle):
+    """Solves of different orders back to back on one handle (its pinned state words,
+    events and grid buffers are reused) give the oracle's bits each time, eigenvalues-only
+    included."""
+    for n, ml, with_q in ((300, 16, True), (96, 4, False), (300, 128, True), (40, 2, True)):
+        d, e = sep.inputs.tridiag_linear(n, seed=n + ml)
+        od, oe = d.copy(), e.copy()
+        oQ = np.asfortranarray(np.eye(n))
+        oracle.qr_algorithm(od, oe, oQ, ml)
+        H = sep.SymTridiagonal(d.copy(), e.copy())
+        Q = np.asfortranarray(np.eye(n)) if with_q else None
+        _, conv, it, nrot = sep.qr_algorithm_(H, Q, ml, handle=handle, return_rotations=True)
+        assert conv
+        assert np.array_equal(H.dv, od) and np.array_equal(H.ev, oe)
+        if with_q:
+            assert np.linalg.norm(Q - oQ) <= 1e-13 * n
+
+
 def test_errors(sep, handle):
     with pytest.raises(AssertionError):
         sep.qr_algorithm_(sep.SymTridiagonal(np.ones(5), np.ones(4)), None, handle=handle)
