@@ -140,6 +140,17 @@ __device__ __forceinline__ double sqrt_common(double t)
 }
 __device__ __forceinline__ float sqrt_common(float t) { return sqrtf(t); }
 
+// fma in T (Float32: one rounding in single precision, as the reference's muladd and the
+// oracle's fmaf; __builtin_fma would widen to double and round twice)
+__device__ __forceinline__ double tfma(double a, double b, double c)
+{
+    return __builtin_fma(a, b, c);
+}
+__device__ __forceinline__ float tfma(float a, float b, float c)
+{
+    return __builtin_fmaf(a, b, c);
+}
+
 template <typename T>
 struct CommonRange;
 template <>
@@ -180,17 +191,33 @@ __device__ __forceinline__ void div2_common(float a, float b, float r, float& qa
 // the numerators (-(f/r) == (-f)/r exactly), and the wave-uniform rare-case test only
 // afterwards redirects to givens_slow, so its branch never stalls the path.
 template <typename T>
-__device__ __forceinline__ Rot<T> givens(T f, T g)
+__device__ __forceinline__ Rot<T> givens_common_path(T f, T g, bool& rare)
 {
     const T af = tabs(f), ag = tabs(g);
-    const T big = __builtin_fmax(af, ag), small = __builtin_fmin(af, ag);
-    const bool rare = !(big < Consts<T>::safmx2) | !(small > CommonRange<T>::lo);
+    // the larger and the smaller magnitude (in T: for Float32 the double fmax would widen
+    // both operands and round the result back)
+    T big, small;
+    if constexpr (std::is_same<T, float>::value) {
+        big = __builtin_fmaxf(af, ag);
+        small = __builtin_fminf(af, ag);
+    } else {
+        big = __builtin_fmax(af, ag);
+        small = __builtin_fmin(af, ag);
+    }
+    rare = !(big < Consts<T>::safmx2) | !(small > CommonRange<T>::lo);
     const bool flip = (af > ag) & (f < T(0));
     const T fs = flip ? -f : f, gs = flip ? -g : g;
     const T r = sqrt_common(f * f + g * g);
     Rot<T> o;
     div2_common(fs, gs, r, o.c, o.s);
     o.r = flip ? -r : r;
+    return o;
+}
+template <typename T>
+__device__ __forceinline__ Rot<T> givens(T f, T g)
+{
+    bool rare;
+    Rot<T> o = givens_common_path(f, g, rare);
     asm volatile("" ::"v"(o.c), "v"(o.s), "v"(o.r));
     if (__builtin_expect(__builtin_amdgcn_readfirstlane((int)rare), 0)) o = givens_slow(f, g);
     return o;
@@ -371,11 +398,24 @@ __device__ __forceinline__ Rot<double> step_rotation(FastRotation, double f, dou
     return givens_fast(f, g, acc);
 }
 
-// The chase step's rotation: with the range test deferred (Float64 chunks) or complete.
+// The chase step's rotation: with the range test deferred (single-solve chunks, and the
+// batched chase's fast form) or complete.
 __device__ __forceinline__ Rot<double> step_rotation(std::true_type, double f, double g,
                                                      uint32_t& acc)
 {
     return givens_deferred(f, g, acc);
+}
+// Float32: givens<float>'s arithmetic, its rare-case flag or-ed into acc (a chunk with
+// acc != 0 is run again with the complete test).  The per-step scalar round trip and
+// branch go, as in Float64: with the single-precision fma and max this takes the n = 2000
+// Float32 solve from 0.19 to 0.14 s (73 instructions per step instead of ~96).
+__device__ __forceinline__ Rot<float> step_rotation(std::true_type, float f, float g,
+                                                    uint32_t& acc)
+{
+    bool rare;
+    const Rot<float> o = givens_common_path(f, g, rare);
+    acc |= (uint32_t)rare;
+    return o;
 }
 template <typename T>
 __device__ __forceinline__ Rot<T> step_rotation(std::false_type, T f, T g, uint32_t&)
@@ -551,13 +591,13 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
     const T H22 = dv[from];
     const Rot<T> G1 = givens(H11 - mu, H21);
     const T c1 = G1.c, s1 = G1.s;
-    const T H11p = __builtin_fma(c1, H11, s1 * H21);
-    const T H21p = __builtin_fma(-s1, H11, c1 * H21);
-    const T H12p = __builtin_fma(c1, H21, s1 * H22);
-    const T H22p = __builtin_fma(-s1, H21, c1 * H22);
-    const T H11pp = __builtin_fma(H11p, c1, H12p * s1);
-    const T H12pp = __builtin_fma(H11p, -s1, H12p * c1);
-    const T H22pp = __builtin_fma(H21p, -s1, H22p * c1);
+    const T H11p = tfma(c1, H11, s1 * H21);
+    const T H21p = tfma(-s1, H11, c1 * H21);
+    const T H12p = tfma(c1, H21, s1 * H22);
+    const T H22p = tfma(-s1, H21, c1 * H22);
+    const T H11pp = tfma(H11p, c1, H12p * s1);
+    const T H12pp = tfma(H11p, -s1, H12p * c1);
+    const T H22pp = tfma(H21p, -s1, H22p * c1);
     T bulge = T(0);
     T e_cur = T(0);  // ev[from+1] after the first step
     if (from + 2 <= to) {
@@ -656,18 +696,18 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                         // 14, as LAPACK's dsteqr sweeps: with X = s (B22 - B11) + 2 c B21,
                         // B11'' = B11 + s X, B22'' = B22 - s X, B12'' = c X - B21 (the
                         // differences are backward stable: errors of eps |B|)
-                        const T X = __builtin_fma(s, B22 - B11, c * (B21 + B21));
-                        B11pp = __builtin_fma(s, X, B11);
-                        B22pp = __builtin_fma(-s, X, B22);
-                        B12pp = __builtin_fma(c, X, -B21);
+                        const T X = tfma(s, B22 - B11, c * (B21 + B21));
+                        B11pp = tfma(s, X, B11);
+                        B22pp = tfma(-s, X, B22);
+                        B12pp = tfma(c, X, -B21);
                     } else {
-                        const T B11p = __builtin_fma(c, B11, s * B21);
-                        const T B21p = __builtin_fma(-s, B11, c * B21);
-                        const T B12p = __builtin_fma(c, B21, s * B22);
-                        const T B22p = __builtin_fma(-s, B21, c * B22);
-                        B11pp = __builtin_fma(B11p, c, B12p * s);
-                        B12pp = __builtin_fma(B11p, -s, B12p * c);
-                        B22pp = __builtin_fma(B21p, -s, B22p * c);
+                        const T B11p = tfma(c, B11, s * B21);
+                        const T B21p = tfma(-s, B11, c * B21);
+                        const T B12p = tfma(c, B21, s * B22);
+                        const T B22p = tfma(-s, B21, c * B22);
+                        B11pp = tfma(B11p, c, B12p * s);
+                        B12pp = tfma(B11p, -s, B12p * c);
+                        B22pp = tfma(B21p, -s, B22p * c);
                     }
                     // every lane writes the same values (the LDS pipe, not the VALU)
                     out[0] = V2{G.r, B11pp};
@@ -686,14 +726,15 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                 // offsets: the compiler otherwise copies the scalar base into a VGPR before
                 // every step's read and writes, two vector moves per step in the batched
                 // chase (46 -> 44 vector instructions; the 4096-problem batch stays within
-                // its 54-56 ms spread).  Not in the single solve's fast chase, whose reads
-                // the compiler then keeps behind the previous steps' writes (a wait per
-                // step instead of two per 32 steps).
+                // its 54-56 ms spread).  Not in the single solve: its fast chase's reads
+                // then stay behind the previous steps' writes (a wait per step instead of
+                // two per 32 steps), and its exact chase, 11 instructions shorter per 32
+                // steps, showed no gain.
                 constexpr int kU = StepUnroll<kSolo>::n;
                 int j = 0;
                 for (; j + kU <= cnt; j += kU) {
                     uint32_t ib = lds_addr(&solo_in[j + 2][0]), ob = lds_addr(&solo_out[j][0]);
-                    if constexpr (!kSolo || !kFast) asm volatile("" : "+v"(ib), "+v"(ob));
+                    if constexpr (!kSolo) asm volatile("" : "+v"(ib), "+v"(ob));
                     const LdsV2* in2 = lds_ptr<V2>(ib);
                     LdsV2* out = lds_ptr<V2>(ob);
 #pragma unroll
@@ -703,12 +744,14 @@ __device__ void single_shift(const Problem<T>& P, int from, int to, T mu, int la
                     step(lds_ptr<V2>(lds_addr(&solo_in[j + 2][0])),
                          lds_ptr<V2>(lds_addr(&solo_out[j][0])));
             };
-            if constexpr (std::is_same<T, double>::value && (kSolo || kFast)) {
+            // (the fast rotation is a Float64 form; Float32 defers the complete test)
+            constexpr bool kF64 = std::is_same<T, double>::value;
+            if constexpr ((kF64 && (kSolo || kFast)) || (!kF64 && kSolo && !kFast)) {
                 if constexpr (kFast)
                     run(FastRotation{});
                 else
                     run(std::true_type{});
-                const uint32_t rare = kFast ? kRareFast : kRareAcc;
+                const uint32_t rare = !kF64 ? 1u : kFast ? kRareFast : kRareAcc;
                 if (__builtin_expect(__builtin_amdgcn_readfirstlane((int)(acc >= rare)), 0)) {
                     p1 = p1_0;
                     d_cur = d_0;

@@ -71,12 +71,15 @@ def test_random_spectrum(sep, handle):
     assert np.linalg.norm(A @ gQ - gQ * gd) <= 1e-13 * np.linalg.norm(A)
 
 
-def test_float32_solve(sep, handle):
-    n = 128
+@pytest.mark.parametrize("n,max_layers", [(128, 64), (500, 16)])
+def test_float32_solve(sep, handle, n, max_layers):
+    """Float32 chase: fma, division and square root in single precision with one rounding
+    each, as the oracle's fmaf, / and sqrtf: dv and ev bit for bit."""
     d, e = sep.inputs.tridiag_linear(n, seed=8)
-    (od, _, oQ, oconv, oit, _), (gd, _, gQ, conv, it, _) = _solve_both(
-        sep, handle, d, e, 64, dtype=np.float32)
+    (od, oe, oQ, oconv, oit, _), (gd, ge, gQ, conv, it, _) = _solve_both(
+        sep, handle, d, e, max_layers, dtype=np.float32)
     assert conv and oconv and it == oit
+    assert np.array_equal(gd, od) and np.array_equal(ge, oe)
     ref = sl.eigvalsh_tridiagonal(d, e)
     assert np.abs(np.sort(gd.astype(np.float64)) - ref).max() <= 1e-5 * np.abs(ref).max()
     A = _dense(d, e)
