@@ -5,7 +5,7 @@ bit for bit, and the layer-major product equals the dense product of the rotatio
 matrices, for random (m, n, k) within the wave schedule's preconditions.
 GPU part (marked gpu): the HIP apply against the oracle for random (m, n, k, lda, element
 type, geometry), and the HIP solve against the oracle bit for bit (dv, ev, iteration and
-rotation counts) for random even orders and block lengths.  Example counts are bounded so
+rotation counts) for random even orders and block lengths, in Float64 and Float32.  Example counts are bounded so
 that each test runs in seconds; the draws are reproducible (derandomized).
 """
 import numpy as np
@@ -167,6 +167,30 @@ def test_gpu_solve_random_bitwise(sep, handle, half_n, max_layers, seed):
     assert conv == oconv and it == oit and nrot == onrot, (n, max_layers, seed)
     assert np.array_equal(H.dv, od) and np.array_equal(H.ev, oe), (n, max_layers, seed)
     assert np.linalg.norm(Q - oQ) <= 1e-13 * np.sqrt(n), (n, max_layers, seed)
+
+
+@pytest.mark.gpu
+@settings(max_examples=60, deadline=None, derandomize=True,
+          suppress_health_check=[HealthCheck.too_slow, HealthCheck.function_scoped_fixture])
+@given(st.integers(1, 90), st.integers(1, 64), st.integers(0, 2 ** 31))
+def test_gpu_solve_random_bitwise_float32(sep, handle, half_n, max_layers, seed):
+    """The same in Float32: the chase's fma, division and square root round once in single
+    precision, as the oracle's fmaf, / and sqrtf, so dv, ev and the counts agree bit for
+    bit; Q to 1e-5 sqrt(n)."""
+    n = 2 * half_n
+    r = np.random.default_rng(seed)
+    d = (np.arange(1.0, n + 1) + r.standard_normal(n)).astype(np.float32)
+    e = r.random(n - 1).astype(np.float32)
+    od, oe = d.copy(), e.copy()
+    oQ = np.asfortranarray(np.eye(n, dtype=np.float32))
+    oconv, oit, onrot, _ = oracle.qr_algorithm(od, oe, oQ, max_layers)
+    H = sep.SymTridiagonal(d.copy(), e.copy())
+    Q = np.asfortranarray(np.eye(n, dtype=np.float32))
+    _, conv, it, nrot = sep.qr_algorithm_(H, Q, max_layers, handle=handle,
+                                         return_rotations=True)
+    assert conv == oconv and it == oit and nrot == onrot, (n, max_layers, seed)
+    assert np.array_equal(H.dv, od) and np.array_equal(H.ev, oe), (n, max_layers, seed)
+    assert np.linalg.norm(Q - oQ) <= 1e-5 * np.sqrt(n), (n, max_layers, seed)
 
 
 @pytest.mark.gpu
