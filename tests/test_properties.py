@@ -5,8 +5,9 @@ bit for bit, and the layer-major product equals the dense product of the rotatio
 matrices, for random (m, n, k) within the wave schedule's preconditions.
 GPU part (marked gpu): the HIP apply against the oracle for random (m, n, k, lda, element
 type, geometry), and the HIP solve against the oracle bit for bit (dv, ev, iteration and
-rotation counts) for random even orders and block lengths, in Float64 and Float32.  Example counts are bounded so
-that each test runs in seconds; the draws are reproducible (derandomized).
+rotation counts) for random even orders and block lengths, in Float64 and Float32.
+Example counts are bounded so that each test runs in seconds; the draws are reproducible
+(derandomized).
 """
 import numpy as np
 import pytest
